@@ -1,0 +1,326 @@
+"""MI355X (gfx950) volumetric hot path of jeanfilho/CloudRendering-Vulkan.
+
+Python mirror of the reference's technique API (RenderTechnique.h:15-66)
+over the C-ABI of libcrmi355x.so.  The host side of the reference is C++;
+this module is the binding the tests, the bench and the driver use.  All
+computation runs in the gfx950 kernels; there is no CPU fallback.
+
+    ctx = Context()
+    ctx.generate_fbm(fbm)                          # or ctx.upload_grid(...)
+    sv = RenderTechniqueSV(ctx); sv.QueueUpdateShadowVolume(svp); sv.RecordDrawCommands()
+    pt = RenderTechniquePT(ctx); pt.QueueUpdateCameraProperties(cam)
+    pt.QueueUpdateParameters(params); pt.UpdateDescriptorSets()
+    pt.RecordDrawCommands(frames)                  # n frames, one persistent launch
+    img = ctx.download_image()
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import structs
+from ._lib import LibraryNotBuilt, load  # noqa: F401
+from .structs import *  # noqa: F401,F403
+
+__all__ = ["Context", "CrError", "RenderTechniquePT", "RenderTechniqueSV", "RenderTechniquePPB", "structs", "load",
+           "LibraryNotBuilt"]
+
+
+class CrError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{message} ({status})")
+        self.status = status
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """Owns one cr_context (device buffers resident in HBM, one HIP stream)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        s = self.lib.cr_create(device, C.byref(h))
+        if s != 0:
+            raise CrError(s, f"cr_create(device={device}) failed: {self.lib.cr_status_string(s).decode()}")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, s, what):
+        if s != 0:
+            msg = self.lib.cr_last_error(self.h).decode()
+            raise CrError(s, f"{what}: {self.lib.cr_status_string(s).decode()}: {msg}")
+
+    @property
+    def stream(self):
+        return self.lib.cr_stream(self.h)
+
+    def synchronize(self):
+        self.check(self.lib.cr_synchronize(self.h), "cr_synchronize")
+
+    # ---------------------------------------------------------------- grid
+    def set_grid_layout(self, layout):
+        self.check(self.lib.cr_set_grid_layout(self.h, layout), "cr_set_grid_layout")
+
+    def upload_grid(self, data, voxel_size, layout=structs.CR_LAYOUT_X_FASTEST, storage=structs.CR_DTYPE_F32):
+        """data: 3-D array indexed [z, y, x] (x fastest) or [x, y, z] with layout Z_FASTEST."""
+        a = np.ascontiguousarray(data)
+        if a.dtype == np.float32:
+            src = structs.CR_DTYPE_F32
+        elif a.dtype == np.float16:
+            src = structs.CR_DTYPE_F16
+        else:
+            raise TypeError("grid must be float32 or float16")
+        if layout == structs.CR_LAYOUT_X_FASTEST:
+            nz, ny, nx = a.shape
+        else:
+            nx, ny, nz = a.shape
+        vs = voxel_size if np.ndim(voxel_size) else (voxel_size,) * 3
+        h = structs.GridHeader(nx, ny, nz, 0, float(vs[0]), float(vs[1]), float(vs[2]))
+        self.check(self.lib.cr_grid_upload(self.h, C.byref(h), _ptr(a), src, layout, storage), "cr_grid_upload")
+
+    def load_xyz(self, path, storage=structs.CR_DTYPE_F32):
+        self.check(self.lib.cr_grid_load_xyz(self.h, path.encode(), storage), "cr_grid_load_xyz")
+
+    def save_xyz(self, path):
+        self.check(self.lib.cr_grid_save_xyz(self.h, path.encode()), "cr_grid_save_xyz")
+
+    def generate_fbm(self, params, storage=structs.CR_DTYPE_F32):
+        self.check(self.lib.cr_grid_generate_fbm(self.h, C.byref(params), storage), "cr_grid_generate_fbm")
+
+    def download_grid(self):
+        c = self.cloud()
+        nx, ny, nz = c.voxelCount[0], c.voxelCount[1], c.voxelCount[2]
+        out = np.empty((nz, ny, nx), np.float32)
+        self.check(self.lib.cr_grid_download(self.h, _ptr(out)), "cr_grid_download")
+        return out
+
+    def majorants(self):
+        d = (C.c_uint32 * 3)()
+        self.check(self.lib.cr_grid_majorants(self.h, d, None), "cr_grid_majorants")
+        out = np.empty((d[2], d[1], d[0]), np.float32)
+        self.check(self.lib.cr_grid_majorants(self.h, d, _ptr(out)), "cr_grid_majorants")
+        return out
+
+    # ------------------------------------------------------------ uniforms
+    def set_camera(self, cam):
+        self.check(self.lib.cr_set_camera(self.h, C.byref(cam)), "cr_set_camera")
+
+    def set_cloud(self, cloud):
+        self.check(self.lib.cr_set_cloud(self.h, C.byref(cloud)), "cr_set_cloud")
+
+    def cloud(self):
+        c = structs.CloudProperties()
+        self.check(self.lib.cr_get_cloud(self.h, C.byref(c)), "cr_get_cloud")
+        return c
+
+    def set_parameters(self, p):
+        self.check(self.lib.cr_set_parameters(self.h, C.byref(p)), "cr_set_parameters")
+
+    def set_shadow_volume(self, s):
+        self.check(self.lib.cr_set_shadow_volume(self.h, C.byref(s)), "cr_set_shadow_volume")
+
+    def set_render_ext(self, e):
+        self.check(self.lib.cr_set_render_ext(self.h, C.byref(e)), "cr_set_render_ext")
+
+    # -------------------------------------------------------- shadow volume
+    def sv_build(self):
+        self.check(self.lib.cr_sv_build(self.h), "cr_sv_build")
+
+    def sv_dims(self):
+        d = (C.c_uint32 * 3)()
+        self.check(self.lib.cr_sv_dims(self.h, d), "cr_sv_dims")
+        return tuple(d)
+
+    def sv_download(self):
+        dx, dy, dz = self.sv_dims()
+        out = np.empty((dz, dy, dx), np.float32)
+        self.check(self.lib.cr_sv_download(self.h, _ptr(out)), "cr_sv_download")
+        return out
+
+    # --------------------------------------------------------- path tracer
+    def clear_image(self):
+        self.check(self.lib.cr_pt_clear(self.h), "cr_pt_clear")
+
+    def pt_render(self, frames):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        self.check(self.lib.cr_pt_render(self.h, arr, len(frames)), "cr_pt_render")
+
+    def image_shape(self, cam):
+        return 2 * cam.halfHeight, 2 * cam.halfWidth
+
+    def download_image(self, H, W):
+        out = np.empty((H, W, 4), np.float32)
+        self.check(self.lib.cr_pt_download(self.h, _ptr(out)), "cr_pt_download")
+        return out
+
+    def upload_image(self, img):
+        a = np.ascontiguousarray(img, dtype=np.float32)
+        self.check(self.lib.cr_pt_upload(self.h, _ptr(a)), "cr_pt_upload")
+
+    def pt_stats(self):
+        s = structs.PtStats()
+        self.check(self.lib.cr_pt_stats_get(self.h, C.byref(s)), "cr_pt_stats_get")
+        return s
+
+    def pt_stats_reset(self):
+        self.check(self.lib.cr_pt_stats_reset(self.h), "cr_pt_stats_reset")
+
+    # -------------------------------------------------------- photon beams
+    def trace_beams(self, params, max_beams):
+        nb, nd = C.c_uint32(), C.c_uint32()
+        self.check(self.lib.cr_ppb_trace_beams(self.h, C.byref(params), max_beams, C.byref(nb), C.byref(nd)),
+                   "cr_ppb_trace_beams")
+        return nb.value, nd.value
+
+    def upload_beams(self, beams, data):
+        self.check(self.lib.cr_beams_upload(self.h, beams, len(beams), data, len(data)), "cr_beams_upload")
+
+    def beams_count(self):
+        n, nd = C.c_uint32(), C.c_uint32()
+        self.check(self.lib.cr_beams_count(self.h, C.byref(n), C.byref(nd)), "cr_beams_count")
+        return n.value, nd.value
+
+    def download_beams(self):
+        n, nd = self.beams_count()
+        beams = (structs.PhotonBeam * n)()
+        data = (structs.PhotonBeamData * nd)()
+        self.check(self.lib.cr_beams_download(self.h, beams, data), "cr_beams_download")
+        return beams, data
+
+    def lbvh_build(self):
+        self.check(self.lib.cr_lbvh_build(self.h), "cr_lbvh_build")
+
+    def lbvh_download(self, nodes=True):
+        n, _ = self.beams_count()
+        keys = np.empty(n, np.uint32)
+        perm = np.empty(n, np.uint32)
+        tn = (structs.TreeNode * max(2 * n - 1, 1))() if nodes else None
+        self.check(self.lib.cr_lbvh_download(self.h, _ptr(keys), _ptr(perm), tn), "cr_lbvh_download")
+        return keys, perm, tn
+
+    def lbvh_timings(self):
+        t = (C.c_double * 5)()
+        self.check(self.lib.cr_lbvh_timings(self.h, t), "cr_lbvh_timings")
+        return dict(zip(["morton", "sort", "hierarchy", "refit", "pack"], list(t)))
+
+    def gather(self, frames, ext=None):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        self.check(self.lib.cr_ppb_gather(self.h, arr, len(frames), C.byref(ext) if ext is not None else None),
+                   "cr_ppb_gather")
+
+    def gather_stats(self):
+        s = structs.GatherStats()
+        self.check(self.lib.cr_gather_stats_get(self.h, C.byref(s)), "cr_gather_stats_get")
+        return s
+
+
+# --------------------------------------------------------------------------
+# Technique objects: the reference's plugin interface (RenderTechnique.h).
+# QueueUpdate* records a binding, UpdateDescriptorSets commits the queue to
+# the context, RecordDrawCommands issues the technique's dispatches.
+# --------------------------------------------------------------------------
+class RenderTechnique:
+    """RenderTechnique.h:15-66 without Vulkan descriptor sets."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self._queue = []
+
+    def QueueUpdateCloudData(self, cloud_properties, imageIdx=0):
+        self._queue.append(("cloud", cloud_properties))
+
+    def QueueUpdateCloudDataSampler(self, grid, imageIdx=0):
+        # grid: dict(data=ndarray[z,y,x], voxel_size=..., storage=...) or None (keep the context grid)
+        if grid is not None:
+            self._queue.append(("grid", grid))
+
+    def QueueUpdateCameraProperties(self, camera, imageIdx=0):
+        self._queue.append(("camera", camera))
+
+    def QueueUpdateParameters(self, parameters, imageIdx=0):
+        self._queue.append(("parameters", parameters))
+
+    def QueueUpdateShadowVolume(self, sv_properties, imageIdx=0):
+        self._queue.append(("shadow_volume", sv_properties))
+
+    def QueueUpdateShadowVolumeSampler(self, _sv_image=None, imageIdx=0):
+        pass  # the shadow volume image is owned by the context
+
+    def UpdateDescriptorSets(self):
+        for kind, value in self._queue:
+            if kind == "cloud":
+                self.ctx.set_cloud(value)
+            elif kind == "grid":
+                self.ctx.upload_grid(value["data"], value["voxel_size"], storage=value.get("storage", 0))
+            elif kind == "camera":
+                self.ctx.set_camera(value)
+            elif kind == "parameters":
+                self.ctx.set_parameters(value)
+            elif kind == "shadow_volume":
+                self.ctx.set_shadow_volume(value)
+        self._queue.clear()
+
+    def GetRequiredSetCount(self):
+        return 1
+
+
+class RenderTechniqueSV(RenderTechnique):
+    """RenderTechniqueSV (RenderTechniqueSV.cpp:102-113): builds the shadow volume."""
+
+    def RecordDrawCommands(self, imageIndex=0):
+        self.UpdateDescriptorSets()
+        self.ctx.sv_build()
+
+
+class RenderTechniquePT(RenderTechnique):
+    """RenderTechniquePT (RenderTechniquePT.cpp:129-168): one call renders len(frames) frames."""
+
+    def RecordDrawCommands(self, frames, imageIndex=0):
+        self.UpdateDescriptorSets()
+        if isinstance(frames, structs.PushConstants):
+            frames = [frames]
+        self.ctx.pt_render(list(frames))
+
+
+class RenderTechniquePPB(RenderTechnique):
+    """RenderTechniquePPB (RenderTechniquePPB.cpp:436-519): beams -> LBVH -> gather."""
+
+    def __init__(self, ctx, initialRadius=0.002):
+        super().__init__(ctx)
+        self.radius = initialRadius
+        self.trace_params = None
+        self.max_beams = 1 << 22
+
+    def AllocateResources(self, trace_params=None, max_beams=1 << 22):
+        self.trace_params = trace_params
+        self.max_beams = max_beams
+
+    def FreeResources(self):
+        pass
+
+    def RecordDrawCommands(self, frames, imageIndex=0, gather_ext=None, retrace=True):
+        self.UpdateDescriptorSets()
+        if retrace and self.trace_params is not None:
+            self.ctx.trace_beams(self.trace_params, self.max_beams)
+        self.ctx.lbvh_build()
+        if isinstance(frames, structs.PushConstants):
+            frames = [frames]
+        self.ctx.gather(list(frames), gather_ext)

@@ -1,0 +1,82 @@
+// Internal interfaces between the host runtime (cr_context.cpp) and the
+// kernel translation units.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cr/abi.h"
+
+namespace cr {
+
+// Device grid descriptor shared by the kernels.
+struct DeviceGrid {
+    void* data = nullptr;       // FP32 or FP16 voxels
+    int dtype = CR_DTYPE_F32;
+    int layout = 0;             // 0 dense x-fastest, 1 4^3 bricks
+    int32_t nx = 0, ny = 0, nz = 0;
+    int32_t nbx = 0, nby = 0, nbz = 0;
+    size_t bytes = 0;
+    float* majorants = nullptr; // ceil(N/16)^3 brick maxima (with apron)
+    int32_t mx = 0, my = 0, mz = 0;
+};
+
+struct PTArgs {
+    // density
+    const void* grid;
+    int32_t nx, ny, nz, nbx, nby;
+    const float* majorants;
+    int32_t mx, my, mz;
+    float bw[3];
+    // shadow volume
+    const float* sv;
+    int32_t svx, svy, svz;
+    // uniforms
+    cr_camera_properties cam;
+    cr_cloud_properties cloud;
+    cr_parameters params;
+    cr_shadow_volume_properties svp;
+    cr_render_ext ext;
+    float sigmaBar;
+    // frames
+    const int32_t* frames;  // pairs (seed, frameCount)
+    uint32_t nFrames;
+    // image
+    float4* image;
+    int32_t W, H;
+    uint32_t padded;        // 32 * (W/32 + 1), the reference's id stride
+    uint32_t tilesX;
+    uint32_t totalIdx;      // tiles * 64
+    uint32_t* queue;        // pixel work counter (zeroed before launch)
+    unsigned long long* counters; // paths, vertices, lookups, svLookups, brickVisits
+};
+
+// kern_pt.hip
+hipError_t launch_pt(const PTArgs& a, int dtype, int layout, int grid_blocks, hipStream_t s);
+int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee);
+
+struct SVArgs {
+    const void* grid;
+    int32_t nx, ny, nz, nbx, nby;
+    cr_cloud_properties cloud;
+    cr_shadow_volume_properties svp;
+    float* out;
+    int32_t dx, dy, dz;
+    float k[3];
+};
+// kern_sv.hip
+hipError_t launch_sv(const SVArgs& a, int dtype, int layout, hipStream_t s);
+
+// kern_grid.hip
+hipError_t launch_fbm(const cr_fbm_params& p, void* out, int dtype, int layout, int32_t nbx, int32_t nby,
+                      hipStream_t s);
+hipError_t launch_convert_grid(const void* src, int src_dtype, int src_layout_zfast, void* dst, int dst_dtype,
+                               int dst_layout, int32_t nx, int32_t ny, int32_t nz, int32_t nbx, int32_t nby,
+                               hipStream_t s);
+hipError_t launch_grid_to_f32(const void* src, int dtype, int layout, float* dst, int32_t nx, int32_t ny, int32_t nz,
+                              int32_t nbx, int32_t nby, hipStream_t s);
+hipError_t launch_majorants(const void* grid, int dtype, int layout, int32_t nx, int32_t ny, int32_t nz, int32_t nbx,
+                            int32_t nby, float* out, int32_t mx, int32_t my, int32_t mz, hipStream_t s);
+hipError_t launch_grid_max(const float* majorants, uint32_t n, unsigned int* out_bits, hipStream_t s);
+
+} // namespace cr

@@ -1,0 +1,91 @@
+// Internal interfaces of the photon-beam kernels (kern_lbvh.hip,
+// kern_ppb.hip) used by cr_runtime_ppb.cpp.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cr/abi.h"
+
+namespace cr {
+
+struct BoxArg {
+    float b0[3];
+    float b1[3];
+};
+
+// One internal node in traversal form: both child boxes and indices.
+struct TravNode {
+    float lo[2][3];
+    float hi[2][3];
+    uint32_t child[2];
+    uint32_t leafMask;  // bit c set: child c is a leaf
+    uint32_t pad;
+};
+static_assert(sizeof(TravNode) == 64, "TravNode is one 64-B line");
+
+hipError_t launch_morton(const cr_photon_beam* beams, uint32_t n, const BoxArg& box, uint32_t* keys, uint32_t* vals,
+                         hipStream_t s);
+uint32_t sort_tiles(uint32_t n);
+hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* counts,
+                             uint32_t n, hipStream_t s);
+hipError_t launch_hierarchy(const uint32_t* keys, uint32_t n, uint32_t* parent, uint32_t* left, uint32_t* right,
+                            hipStream_t s);
+hipError_t launch_leaves(const cr_photon_beam* beams, const uint32_t* perm, const uint32_t* keys, uint32_t n,
+                         cr_photon_beam* sorted, float* lo, float* hi, hipStream_t s);
+hipError_t launch_refit(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, float* lo, float* hi,
+                        uint32_t* arrivals, hipStream_t s);
+hipError_t launch_pack(uint32_t n, const uint32_t* left, const uint32_t* right, const float* lo, const float* hi,
+                       TravNode* out, hipStream_t s);
+hipError_t launch_treenodes(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, const float* lo,
+                            const float* hi, cr_tree_node* out, hipStream_t s);
+
+// Beam producer: the scene a photon path sees.
+struct TraceArgs {
+    const void* grid;
+    int32_t nx, ny, nz, nbx, nby;
+    cr_cloud_properties cloud;
+    cr_parameters params;
+    cr_beam_trace_params p;
+    // write pass only
+    const uint64_t* segOffsets;
+    const uint64_t* dataOffsets;
+    cr_photon_beam* beams;
+    cr_photon_beam_data* data;
+    uint64_t maxBeams;
+    uint64_t maxData;
+    // count pass only
+    uint32_t* segCounts;
+    uint32_t* dataCounts;
+};
+hipError_t launch_trace_count(const TraceArgs& a, int dtype, int layout, hipStream_t s);
+hipError_t launch_trace_write(const TraceArgs& a, int dtype, int layout, hipStream_t s);
+
+struct GatherArgs {
+    const void* grid;
+    int32_t nx, ny, nz, nbx, nby;
+    const float* sv;
+    int32_t svx, svy, svz;
+    cr_camera_properties cam;
+    cr_cloud_properties cloud;
+    cr_parameters params;
+    cr_shadow_volume_properties svp;
+    const cr_photon_beam* beams;     // sorted
+    const cr_photon_beam_data* data;
+    const TravNode* nodes;
+    const float* leafLo;             // box of node 0 when n == 1
+    const float* leafHi;
+    uint32_t n;
+    uint32_t refCompat, jitter;
+    float albedo;
+    const int32_t* frames;
+    uint32_t nFrames;
+    float4* image;
+    int32_t W, H;
+    uint32_t padded;
+    unsigned long long* counters;    // rays, nodesVisited, beamsTested
+    uint32_t* overflow;              // set when a traversal stack would overflow
+};
+hipError_t launch_gather(const GatherArgs& a, int dtype, int layout, hipStream_t s);
+
+} // namespace cr

@@ -1,0 +1,70 @@
+// The context behind the opaque cr_context handle, shared by the runtime
+// translation units.  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "cr_internal.h"
+
+struct cr_context {
+    int device = 0;
+    int numCUs = 0;
+    std::string arch;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // path tracer / shadow volume launches
+    hipEvent_t gev0 = nullptr, gev1 = nullptr; // gather launches
+    std::string err;
+
+    // density grid
+    cr::DeviceGrid grid;
+    int gridLayout = CR_GRID_DENSE;
+    bool hasGrid = false;
+    float gridMax = 0.0f;
+    double voxelSize[3] = {0, 0, 0};
+
+    // uniforms (RenderTechnique descriptor bindings)
+    cr_camera_properties cam;
+    cr_cloud_properties cloud;
+    cr_parameters params;
+    cr_shadow_volume_properties svp;
+    cr_render_ext ext;
+
+    // shadow volume
+    float* sv = nullptr;
+    size_t svCap = 0;
+    bool svValid = false;
+    uint32_t svDimsBuilt[3] = {0, 0, 0};
+    float svMs = 0.0f;
+
+    // accumulation image (RGBA32F)
+    float* image = nullptr;
+    size_t imageCap = 0;
+    int W = 0, H = 0;
+
+    // path tracer
+    int32_t* dFrames = nullptr;
+    size_t framesCap = 0;
+    uint32_t* dQueue = nullptr;
+    size_t queueCap = 0;
+    unsigned long long* dCounters = nullptr;
+    size_t countersCap = 0;
+    void* dScalar = nullptr;
+    size_t scalarCap = 0;
+    bool ptPending = false;
+    float ptLastMs = 0.0f;
+    int ptBlocksOverride = 0;
+
+    // photon beams / LBVH (cr_runtime_ppb.cpp)
+    struct Ppb* ppb = nullptr;
+};
+
+namespace cr {
+cr_status fail(cr_context* ctx, cr_status code, const std::string& msg);
+cr_status hip_fail(cr_context* ctx, hipError_t e, const char* what);
+cr_status ensure_buffer(cr_context* ctx, void** p, size_t* cap, size_t bytes, const char* what);
+hipError_t zero_async(void* p, size_t bytes, hipStream_t s);
+cr_status upload_frames(cr_context* ctx, const cr_push_constants* frames, uint32_t n);
+void ppb_free(cr_context* ctx);
+} // namespace cr

@@ -1,0 +1,379 @@
+// Photon-beam LBVH build for gfx950: Morton codes, stable LSD radix sort,
+// Karras hierarchy and bottom-up AABB refit — the method of the reference's
+// PPB_PT.comp:317-333 (Morton), PPB_RadixSort_{LocalSort,PrefixSum,
+// GlobalSort}.comp (sort), PPB_GenerateHierarchy.comp (hierarchy) and
+// PPB_CalculateAABB.comp (refit), with the corrections D13-D22.
+//
+// Radix sort: 4 passes of 8-bit digits.  Each pass is
+//   (1) tile histograms: 4096 keys per 256-thread block, per-wave LDS
+//       histograms, written digit-major (digit * tiles + tile);
+//   (2) one exclusive scan over the 256 x tiles counts (single block);
+//   (3) stable scatter: the tile is consumed in rounds of 256 keys (one key
+//       per lane, coalesced); within a wave, lanes with equal digits are found
+//       with 8 ballots (a wave-level multisplit), their rank is the popcount
+//       of equal-digit lanes below; across the 4 waves of the round and the
+//       previous rounds of the tile, ranks are offset by LDS counters.  The
+//       result is the unique stable order "by key, ties by input index".
+// Hierarchy: one lane per internal node, Karras 2012 Fig. 4 with the
+// index tie-break for equal codes.  Refit: one lane per leaf walks up; the
+// second lane to reach a node (agent-scope acq_rel counter) unions the two
+// child boxes.
+#include "cr_device.h"
+#include "cr_internal_ppb.h"
+
+using namespace crd;
+
+namespace {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int TILE = 4096;
+constexpr int ROUNDS = TILE / 256;
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v)
+{
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t morton30(f3 c, const float* b0, const float* b1)
+{
+    uint32_t q[3];
+    const float cc[3] = {c.x, c.y, c.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float n = fdiv(cc[k] - b0[k], b1[k] - b0[k]);
+        q[k] = (uint32_t)fminf(fmaxf(n * 1024.0f, 0.0f), 1023.0f);
+    }
+    return (expand_bits10(q[0]) << 2) + (expand_bits10(q[1]) << 1) + expand_bits10(q[2]);
+}
+
+__global__ void __launch_bounds__(256) morton_kernel(const cr_photon_beam* beams, uint32_t n, cr::BoxArg box, uint32_t* keys,
+                                                     uint32_t* vals)
+{
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4* p = reinterpret_cast<const float4*>(beams + i);
+    float4 a = p[0];  // startPos, mortonCode
+    float4 b = p[1];  // endPos, radius
+    f3 c = mk(fdiv(a.x + b.x, 2.0f), fdiv(a.y + b.y, 2.0f), fdiv(a.z + b.z, 2.0f));
+    keys[i] = morton30(c, box.b0, box.b1);
+    vals[i] = i;
+}
+
+// (1) per-tile digit histograms, digit-major.
+__global__ void __launch_bounds__(256) hist_kernel(const uint32_t* keys, uint32_t n, uint32_t shift, uint32_t tiles,
+                                                   uint32_t* counts)
+{
+    __shared__ uint32_t h[4][256];
+    const uint32_t w = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < 1024; i += 256) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * TILE;
+#pragma unroll 4
+    for (int r = 0; r < ROUNDS; r++) {
+        uint32_t i = base + (uint32_t)r * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[w][(keys[i] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    const uint32_t d = threadIdx.x;
+    counts[(size_t)d * tiles + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+}
+
+// (2) exclusive scan of m counts in place, one block of 1024 threads.
+__global__ void __launch_bounds__(1024) scan_kernel(uint32_t* c, uint32_t m)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (m + 1023) / 1024;
+    const uint32_t lo = t * per, hi = min(lo + per, m);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; i++) s += c[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;  // exclusive prefix of this thread's chunk
+    for (uint32_t i = lo; i < hi; i++) {
+        uint32_t v = c[i];
+        c[i] = run;
+        run += v;
+    }
+}
+
+// (3) stable scatter of one tile.
+__global__ void __launch_bounds__(256) scatter_kernel(const uint32_t* keys, const uint32_t* vals, uint32_t n, uint32_t shift,
+                                                      uint32_t tiles, const uint32_t* offsets, uint32_t* keys_out,
+                                                      uint32_t* vals_out)
+{
+    __shared__ uint32_t run[256];      // keys of each digit already placed by earlier rounds
+    __shared__ uint32_t wcnt[4][256];  // per-wave digit counts of the current round
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63u;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    run[t] = offsets[(size_t)t * tiles + blockIdx.x];
+    for (uint32_t i = t; i < 1024; i += 256) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * TILE;
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t i = base + (uint32_t)r * 256 + t;
+        const bool valid = i < n;
+        uint32_t k = 0, v = 0, d = 0;
+        if (valid) {
+            k = keys[i];
+            v = vals[i];
+            d = (k >> shift) & 0xffu;
+        }
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (uint32_t q = 0; q < w; q++) pos += wcnt[q][d];
+            keys_out[pos] = k;
+            vals_out[pos] = v;
+        }
+        __syncthreads();
+        run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        wcnt[0][t] = 0; wcnt[1][t] = 0; wcnt[2][t] = 0; wcnt[3][t] = 0;
+        __syncthreads();
+    }
+}
+
+// delta(i, j) of Karras 2012 with the index tie-break (D18, D19).
+__device__ __forceinline__ int delta(const uint32_t* k, int32_t n, int32_t i, int32_t j)
+{
+    if (j < 0 || j > n - 1) return -1;
+    const uint32_t a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+    return __clz((int)(a ^ b));
+}
+
+__global__ void __launch_bounds__(256) hierarchy_kernel(const uint32_t* k, int32_t n, uint32_t* parent, uint32_t* left,
+                                                        uint32_t* right)
+{
+    const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= n - 1) return;
+    const int d = (delta(k, n, i, i + 1) - delta(k, n, i, i - 1)) > 0 ? 1 : -1;
+    const int dmin = delta(k, n, i, i - d);
+    int32_t lmax = 2;
+    while (delta(k, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int32_t l = 0;
+    for (int32_t t = lmax / 2; t >= 1; t /= 2)
+        if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    const int32_t j = i + l * d;
+    const int32_t first = min(i, j), last = max(i, j);
+    // findSplit on (first, last) (D20)
+    const int dnode = delta(k, n, first, last);
+    int32_t s = first, step = last - first;
+    do {
+        step = (step + 1) >> 1;
+        const int32_t ns = s + step;
+        if (ns < last && delta(k, n, first, ns) > dnode) s = ns;
+    } while (step > 1);
+    const uint32_t inner = (uint32_t)(n - 1);
+    const uint32_t lc = (first == s) ? inner + (uint32_t)s : (uint32_t)s;
+    const uint32_t rc = (last == s + 1) ? inner + (uint32_t)(s + 1) : (uint32_t)(s + 1);
+    left[i] = lc;
+    right[i] = rc;
+    parent[lc] = (uint32_t)i;
+    parent[rc] = (uint32_t)i;
+    if (i == 0) parent[0] = NONE;
+}
+
+// Reorder the beams into sorted order (the reference's sort moves whole
+// PhotonBeam records) and write each leaf's box (calculateBeamBounds,
+// PPB_CalculateAABB.comp:64-80; a zero-length beam gets the ball of its radius).
+__global__ void __launch_bounds__(256) leaf_kernel(const cr_photon_beam* beams, const uint32_t* perm, const uint32_t* keys,
+                                                   uint32_t n, cr_photon_beam* sorted, float* lo, float* hi)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    cr_photon_beam b = beams[perm[i]];
+    b.mortonCode = keys[i];
+    sorted[i] = b;
+    f3 pa = ld3(b.startPos), pb = ld3(b.endPos);
+    f3 a = sub(pb, pa);
+    float dd = dot(a, a);
+    float ra = b.radius;
+    f3 e;
+    if (dd > 0.0f) {
+        e = mk(ra * fsqrt(1.0f - fdiv(a.x * a.x, dd)), ra * fsqrt(1.0f - fdiv(a.y * a.y, dd)),
+               ra * fsqrt(1.0f - fdiv(a.z * a.z, dd)));
+    } else {
+        e = mk(ra, ra, ra);
+    }
+    const size_t L = (size_t)(n - 1) + i;
+    lo[3 * L + 0] = fminf(pa.x - e.x, pb.x - e.x);
+    lo[3 * L + 1] = fminf(pa.y - e.y, pb.y - e.y);
+    lo[3 * L + 2] = fminf(pa.z - e.z, pb.z - e.z);
+    hi[3 * L + 0] = fmaxf(pa.x + e.x, pb.x + e.x);
+    hi[3 * L + 1] = fmaxf(pa.y + e.y, pb.y + e.y);
+    hi[3 * L + 2] = fmaxf(pa.z + e.z, pb.z + e.z);
+}
+
+// Bottom-up union (PPB_CalculateAABB.comp:100-120, corrected D22): the first
+// lane to arrive at a node stops; the second unions both children.  The
+// counter RMW is acq_rel at agent scope, so the box stores of the first lane
+// (made before its release) are visible to the second lane's loads (made
+// after its acquire), on every XCD.
+__global__ void __launch_bounds__(256) refit_kernel(uint32_t n, const uint32_t* parent, const uint32_t* left,
+                                                    const uint32_t* right, float* lo, float* hi, uint32_t* arrivals)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    uint32_t cur = parent[(n - 1) + i];
+    // A Karras tree over 30-bit codes with index tie-breaks is at most 64
+    // levels deep; the bound only protects against a malformed tree.
+    for (int level = 0; cur != NONE && cur < n - 1 && level < 128; level++) {
+        const uint32_t old = __hip_atomic_fetch_add(arrivals + cur, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) return;
+        const uint32_t a = left[cur], b = right[cur];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            lo[3 * (size_t)cur + k] = fminf(lo[3 * (size_t)a + k], lo[3 * (size_t)b + k]);
+            hi[3 * (size_t)cur + k] = fmaxf(hi[3 * (size_t)a + k], hi[3 * (size_t)b + k]);
+        }
+        cur = parent[cur];
+    }
+}
+
+// Traversal layout: one 64-B record per internal node holding both child
+// boxes and child indices, so one traversal step reads one cache line.
+__global__ void __launch_bounds__(256) pack_kernel(uint32_t n, const uint32_t* left, const uint32_t* right, const float* lo,
+                                                   const float* hi, cr::TravNode* out)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i + 1 >= n) return;
+    const uint32_t a = left[i], b = right[i];
+    cr::TravNode t;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        t.lo[0][k] = lo[3 * (size_t)a + k];
+        t.hi[0][k] = hi[3 * (size_t)a + k];
+        t.lo[1][k] = lo[3 * (size_t)b + k];
+        t.hi[1][k] = hi[3 * (size_t)b + k];
+    }
+    t.child[0] = a;
+    t.child[1] = b;
+    t.leafMask = (a >= n - 1 ? 1u : 0u) | (b >= n - 1 ? 2u : 0u);
+    t.pad = 0;
+    out[i] = t;
+}
+
+// Reference TreeNode layout (UniformBuffers.h:179-188) for download.
+__global__ void __launch_bounds__(256) treenode_kernel(uint32_t n, const uint32_t* parent, const uint32_t* left,
+                                                       const uint32_t* right, const float* lo, const float* hi,
+                                                       cr_tree_node* out)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t total = 2 * n - 1;
+    if (i >= total) return;
+    const uint32_t inner = n - 1;
+    cr_tree_node t;
+    t.parent = parent[i];
+    const bool leaf = i >= inner;
+    t.left = leaf ? i - inner : left[i];
+    t.right = leaf ? NONE : right[i];
+    t.isLeaf = leaf ? 1u : 0u;
+    for (int k = 0; k < 3; k++) {
+        t.bounds[0][k] = lo[3 * (size_t)i + k];
+        t.bounds[1][k] = hi[3 * (size_t)i + k];
+    }
+    t.bounds[0][3] = 0.0f;
+    t.bounds[1][3] = 0.0f;
+    t.processed = leaf ? 0u : 2u;
+    t._padding_node[0] = t._padding_node[1] = t._padding_node[2] = 0u;
+    out[i] = t;
+}
+
+inline unsigned blocks_for(uint64_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+} // namespace
+
+namespace cr {
+
+hipError_t launch_morton(const cr_photon_beam* beams, uint32_t n, const BoxArg& box, uint32_t* keys, uint32_t* vals,
+                         hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, beams, n, box, keys, vals);
+    return hipGetLastError();
+}
+
+uint32_t sort_tiles(uint32_t n) { return (n + TILE - 1) / TILE; }
+
+hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* counts,
+                             uint32_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t tiles = sort_tiles(n);
+    uint32_t* ki = keys;
+    uint32_t* vi = vals;
+    uint32_t* ko = keys_tmp;
+    uint32_t* vo = vals_tmp;
+    for (uint32_t pass = 0; pass < 4; pass++) {
+        const uint32_t shift = 8u * pass;
+        hipLaunchKernelGGL(hist_kernel, dim3(tiles), dim3(256), 0, s, ki, n, shift, tiles, counts);
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, counts, tiles * 256u);
+        hipLaunchKernelGGL(scatter_kernel, dim3(tiles), dim3(256), 0, s, ki, vi, n, shift, tiles, counts, ko, vo);
+        uint32_t* t = ki; ki = ko; ko = t;
+        t = vi; vi = vo; vo = t;
+    }
+    // 4 passes: the result is back in keys/vals
+    return hipGetLastError();
+}
+
+hipError_t launch_hierarchy(const uint32_t* keys, uint32_t n, uint32_t* parent, uint32_t* left, uint32_t* right,
+                            hipStream_t s)
+{
+    if (n < 2) return hipSuccess;
+    hipLaunchKernelGGL(hierarchy_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, keys, (int32_t)n, parent, left, right);
+    return hipGetLastError();
+}
+
+hipError_t launch_leaves(const cr_photon_beam* beams, const uint32_t* perm, const uint32_t* keys, uint32_t n,
+                         cr_photon_beam* sorted, float* lo, float* hi, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(leaf_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, beams, perm, keys, n, sorted, lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_refit(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, float* lo, float* hi,
+                        uint32_t* arrivals, hipStream_t s)
+{
+    if (n < 2) return hipSuccess;
+    hipLaunchKernelGGL(refit_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, parent, left, right, lo, hi, arrivals);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(uint32_t n, const uint32_t* left, const uint32_t* right, const float* lo, const float* hi,
+                       TravNode* out, hipStream_t s)
+{
+    if (n < 2) return hipSuccess;
+    hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, n, left, right, lo, hi, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_treenodes(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, const float* lo,
+                            const float* hi, cr_tree_node* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(treenode_kernel, dim3(blocks_for(2ull * n - 1, 256)), dim3(256), 0, s, n, parent, left, right, lo, hi, out);
+    return hipGetLastError();
+}
+
+} // namespace cr

@@ -1,0 +1,276 @@
+/*
+ * C-ABI of the MI355X (gfx950) volumetric hot path.
+ *
+ * The reference drives its GPU work through the RenderTechnique plugin API
+ * (CloudRendering-Vulkan/RenderTechnique.h:15-66): QueueUpdate* binds the
+ * uniform buffers and images, UpdateDescriptorSets commits them and
+ * RecordDrawCommands records the dispatches.  This header is the same
+ * contract without Vulkan: plain extern "C" functions, plain pointers and
+ * sizes, status codes instead of abort()/exceptions, one opaque context per
+ * host thread (a context is not thread-safe; contexts are independent).
+ *
+ * Ownership: the caller owns every host buffer it passes; the library owns
+ * the device buffers (density grid, shadow volume, accumulation image, beams,
+ * tree, network weights) and keeps them resident in HBM between calls.
+ * Device work is issued on the context's own HIP stream; functions that hand
+ * results back to host memory synchronise that stream first.
+ *
+ * Everything marked "extension" has no counterpart in the reference; with
+ * the defaults (cr_render_ext_default) every path reduces to the reference's
+ * behaviour.
+ */
+#ifndef CR_ABI_H
+#define CR_ABI_H
+
+#include "uniforms.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(_WIN32)
+#define CR_API __declspec(dllexport)
+#else
+#define CR_API __attribute__((visibility("default")))
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef int32_t cr_status;
+enum {
+    CR_OK = 0,
+    CR_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad size, value out of range */
+    CR_ERR_OUT_OF_MEMORY = 2,
+    CR_ERR_DEVICE = 3,           /* HIP runtime error (message in cr_last_error) */
+    CR_ERR_CAPACITY = 4,         /* output would exceed a stated capacity */
+    CR_ERR_IO = 5,               /* file missing, short or malformed */
+    CR_ERR_STATE = 6,            /* a prerequisite was not set (no grid, no tree, ...) */
+    CR_ERR_NOT_SUPPORTED = 7
+};
+
+#define CR_UNBOUNDED 0xFFFFFFFFu
+
+typedef struct cr_context cr_context;
+
+/* Human-readable name of a status code (static storage). */
+CR_API const char* cr_status_string(cr_status status);
+/* Last error message recorded on this context (valid until the next call). */
+CR_API const char* cr_last_error(const cr_context* ctx);
+/* Library version string, e.g. "cr-mi355x 0.1 gfx950". */
+CR_API const char* cr_version(void);
+
+/* ---------------------------------------------------------------- context
+ * Replaces the reference's device/technique construction
+ * (main.cpp:765-910 InitializeVulkan, RenderTechniquePT.cpp:11-50). */
+CR_API cr_status cr_create(int device, cr_context** out_ctx);
+CR_API cr_status cr_destroy(cr_context* ctx);
+/* Wait for all work issued on the context's stream. */
+CR_API cr_status cr_synchronize(cr_context* ctx);
+/* The context's hipStream_t, for callers that time or order work around it. */
+CR_API void* cr_stream(cr_context* ctx);
+
+/* ------------------------------------------------------- host-side setters
+ * Restatements of the reference's struct setters, so a caller can build the
+ * uniform structs without glm.  They touch host memory only. */
+/* UniformBuffers.h:15-27 default members. */
+CR_API void cr_camera_default(cr_camera_properties* cam);
+/* UniformBuffers.h:46-50 CameraProperties::SetResolution. */
+CR_API void cr_camera_set_resolution(cr_camera_properties* cam, int width, int height);
+/* UniformBuffers.h:52-58 CameraProperties::SetFOV; *fov is clamped to [60,150] in place. */
+CR_API void cr_camera_set_fov(cr_camera_properties* cam, float* fov_degrees);
+/* UniformBuffers.h:29-35 CameraProperties::SetRotation; rotation[0] clamped to [-80,80]. */
+CR_API void cr_camera_set_rotation(cr_camera_properties* cam, float rotation_degrees[2]);
+/* UniformBuffers.h:80-91 default Parameters. */
+CR_API void cr_parameters_default(cr_parameters* p);
+/* UniformBuffers.h:93-106 Parameters::SetPhaseG (ignores |g|>1, clamps to +-0.95). */
+CR_API void cr_parameters_set_phase_g(cr_parameters* p, float g);
+/* UniformBuffers.h:71-78 default CloudProperties. */
+CR_API void cr_cloud_default(cr_cloud_properties* c);
+/* UniformBuffers.h:114-127 default ShadowVolumeProperties. */
+CR_API void cr_shadow_volume_default(cr_shadow_volume_properties* sv);
+/* UniformBuffers.h:131-143 SetLightDirection then UniformBuffers.h:145-151 SetOrigin,
+ * in the order main.cpp:223-236 calls them (the pair fixes deviation D9). */
+CR_API void cr_shadow_volume_set_light(cr_shadow_volume_properties* sv,
+                                       const float light_dir[3],
+                                       const float lower_bound[3],
+                                       const float upper_bound[3]);
+/* MSVC rand() after srand(seed): the per-frame seed sequence of main.cpp:722/:949. */
+CR_API void cr_msvc_rand_sequence(uint32_t srand_seed, uint32_t count, int32_t* out);
+
+/* ------------------------------------------------------------------- grid
+ * Grid3D / .xyz (Grid3D.h:44-192, GridData.h:1-25, main.cpp:144-190). */
+typedef struct cr_grid_header {
+    uint32_t sizeX, sizeY, sizeZ;
+    uint32_t _pad;
+    double voxelSizeX, voxelSizeY, voxelSizeZ;
+} cr_grid_header;
+
+enum { CR_DTYPE_F32 = 0, CR_DTYPE_F16 = 1 };
+enum { CR_LAYOUT_X_FASTEST = 0 /* Grid3D memory order */, CR_LAYOUT_Z_FASTEST = 1 /* .xyz file order */ };
+
+/* Upload a density grid.  `data` holds sizeX*sizeY*sizeZ values of
+ * `src_dtype` in `layout`; it is stored on the device as `storage_dtype`
+ * (FP16 storage is an extension; values convert round-to-nearest-even).
+ * Applies SetCloudProperties (main.cpp:144-163) with the context's current
+ * baseScaling: bounds from voxel sizes, voxelCount, maxExtinction =
+ * max(grid maximum, 0.01).  Also builds the 16^3 brick-majorant grid. */
+CR_API cr_status cr_grid_upload(cr_context* ctx, const cr_grid_header* hdr, const void* data,
+                                int src_dtype, int layout, int storage_dtype);
+/* Device layout used by the next upload or generation (extension):
+ * CR_GRID_DENSE = x-fastest as Grid3D memory; CR_GRID_BRICK4 = 4x4x4-voxel
+ * bricks (128 B per FP16 brick) stored brick-major, so a trilinear footprint
+ * usually falls in one or two cache lines.  Results do not depend on it. */
+enum { CR_GRID_DENSE = 0, CR_GRID_BRICK4 = 1 };
+CR_API cr_status cr_set_grid_layout(cr_context* ctx, int layout);
+/* Read an .xyz file (Grid3D<float>::Load semantics: z-fastest file, x-fastest memory). */
+CR_API cr_status cr_grid_load_xyz(cr_context* ctx, const char* path, int storage_dtype);
+/* Write the current grid as .xyz in file order (z fastest), so load(save(g)) == g. */
+CR_API cr_status cr_grid_save_xyz(cr_context* ctx, const char* path);
+/* Copy the device grid back as FP32, x fastest (sizeX*sizeY*sizeZ floats). */
+CR_API cr_status cr_grid_download(cr_context* ctx, float* out);
+/* Brick-majorant grid (extension): ceil(N/16) per axis, x fastest, max density
+ * over each 16^3 brick plus a one-voxel apron. */
+CR_API cr_status cr_grid_majorants(cr_context* ctx, uint32_t dims_out[3], float* out /* may be NULL */);
+
+/* Seeded value-noise fBm cloud generated on the device (extension; the
+ * reference ships no clouds, models/PUT_YOUR_MODELS_HERE.txt). */
+typedef struct cr_fbm_params {
+    uint32_t size[3];
+    float voxelSize;       /* world units per voxel (isotropic) */
+    uint32_t seed;
+    uint32_t octaves;
+    float lacunarity;
+    float gain;
+    float baseFrequency;   /* lattice cells per world unit at octave 0 */
+    uint32_t shape;        /* CR_SHAPE_* */
+    float thresholdLo;     /* smoothstep edges applied to fbm * shape */
+    float thresholdHi;
+} cr_fbm_params;
+enum { CR_SHAPE_CUMULUS = 0, CR_SHAPE_ANVIL = 1 };
+CR_API void cr_fbm_params_default(cr_fbm_params* p);
+CR_API cr_status cr_grid_generate_fbm(cr_context* ctx, const cr_fbm_params* p, int storage_dtype);
+
+/* --------------------------------------------------------------- uniforms
+ * RenderTechnique::QueueUpdate{CameraProperties,CloudData,Parameters,
+ * ShadowVolume} + UpdateDescriptorSets (RenderTechnique.h:41-46). */
+CR_API cr_status cr_set_camera(cr_context* ctx, const cr_camera_properties* cam);
+CR_API cr_status cr_set_cloud(cr_context* ctx, const cr_cloud_properties* cloud);
+CR_API cr_status cr_get_cloud(cr_context* ctx, cr_cloud_properties* cloud);
+CR_API cr_status cr_set_parameters(cr_context* ctx, const cr_parameters* params);
+CR_API cr_status cr_set_shadow_volume(cr_context* ctx, const cr_shadow_volume_properties* sv);
+
+/* Render extensions (none exist in the reference). */
+typedef struct cr_render_ext {
+    float albedo;          /* scattering albedo; 1 = reference (deviation D1) */
+    uint32_t bounceCap;    /* real collisions that scatter; CR_UNBOUNDED = reference */
+    uint32_t skyMode;      /* CR_SKY_* */
+    float skyColor[4];     /* radiance of the constant sky */
+    uint32_t neeMode;      /* CR_NEE_* */
+    uint32_t trackingMode; /* CR_TRACK_* */
+    uint32_t svDims[3];    /* shadow-volume texels per axis; 0,0,0 = voxelAxisCount^3 */
+    uint32_t _reserved[4];
+} cr_render_ext;
+enum { CR_SKY_GRADIENT = 0 /* PathTracer.comp:104-113 */, CR_SKY_CONSTANT = 1 };
+enum { CR_NEE_SHADOW_VOLUME = 0 /* PathTracer.comp:405-411 */, CR_NEE_RATIO_TRACKING = 1, CR_NEE_NONE = 2 };
+enum { CR_TRACK_GLOBAL_MAJORANT = 0 /* PathTracer.comp:335-355 */, CR_TRACK_BRICK_DDA = 1 };
+CR_API void cr_render_ext_default(cr_render_ext* ext);
+CR_API cr_status cr_set_render_ext(cr_context* ctx, const cr_render_ext* ext);
+
+/* ----------------------------------------------------------- shadow volume
+ * RenderTechniqueSV::RecordDrawCommands (RenderTechniqueSV.cpp:102-113) running
+ * shaders/ShadowVolume.comp: light-space transmittance volume. */
+CR_API cr_status cr_sv_build(cr_context* ctx);
+CR_API cr_status cr_sv_dims(cr_context* ctx, uint32_t dims_out[3]);
+CR_API cr_status cr_sv_download(cr_context* ctx, float* out /* dims x*y*z, x fastest */);
+
+/* ------------------------------------------------------------ path tracer
+ * RenderTechniquePT::RecordDrawCommands (RenderTechniquePT.cpp:129-168) running
+ * shaders/PathTracer.comp, with the blit/present dropped.  The accumulation
+ * image is RGBA32F, width x height = 2*halfWidth x 2*halfHeight of the camera,
+ * row-major from the top-left pixel (the reference's image coordinates).
+ * One call renders n_frames frames in one persistent launch; frame f uses
+ * frames[f].seed and blends with the running mean
+ *     acc = (L + acc * frameCount) / (frameCount + 1)       (PathTracer.comp:420-424)
+ * exactly as n_frames consecutive reference frames would. */
+CR_API cr_status cr_pt_clear(cr_context* ctx);
+CR_API cr_status cr_pt_render(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames);
+CR_API cr_status cr_pt_download(cr_context* ctx, float* rgba);
+CR_API cr_status cr_pt_upload(cr_context* ctx, const float* rgba);
+
+typedef struct cr_pt_stats {
+    uint64_t paths;      /* camera paths traced (one per pixel per frame) */
+    uint64_t vertices;   /* camera-ray origins + accepted (real) collisions */
+    uint64_t lookups;    /* density lookups (primary tracking + shadow rays) */
+    uint64_t svLookups;  /* shadow-volume lookups */
+    uint64_t brickVisits;/* brick-majorant cells entered (DDA tracking) */
+    double lastKernelMs; /* duration of the last render launch (HIP events) */
+} cr_pt_stats;
+CR_API cr_status cr_pt_stats_get(cr_context* ctx, cr_pt_stats* out);
+CR_API cr_status cr_pt_stats_reset(cr_context* ctx);
+
+
+/* ------------------------------------------------------------ photon beams
+ * RenderTechniquePPB (RenderTechniquePPB.h:7-129): beam production
+ * (shaders/PPB_PT.comp), the LBVH build (Morton codes, radix sort,
+ * PPB_GenerateHierarchy.comp, PPB_CalculateAABB.comp) and the gather
+ * (PPB_PE.comp).  The reference builds these pipelines but dispatches only the
+ * tracer and one local-sort pass (RenderTechniquePPB.cpp:436-519); here every
+ * stage runs. */
+typedef struct cr_beam_trace_params {
+    uint32_t numPaths;        /* photon paths (one per thread; id = path index) */
+    int32_t seed;             /* initializeRandom(seed * id) as PPB_PT.comp:513 */
+    float radius;             /* beam radius (pmRadius, fixed) */
+    float albedo;             /* continuation probability at a real collision (1 - absorption) */
+    float lightDirection[4];  /* propagation direction of sunlight (normalised) */
+    float lightIntensity;
+    uint32_t maxInteractions; /* cap on scattering events per path */
+    uint32_t _reserved[2];
+} cr_beam_trace_params;
+CR_API void cr_beam_trace_params_default(cr_beam_trace_params* p);
+
+/* Trace sun photon paths through the current grid and deposit beams, in path
+ * order (deterministic: a counting pass, a scan and a writing pass replace
+ * the reference's atomic appends, PPB_PT.comp:340, :381).  Keeps at most
+ * max_beams segments; *out_beams and *out_data receive the counts kept. */
+CR_API cr_status cr_ppb_trace_beams(cr_context* ctx, const cr_beam_trace_params* p, uint32_t max_beams,
+                                    uint32_t* out_beams, uint32_t* out_data);
+/* Replace the device beams with caller data (n beams, n_data PhotonBeamData). */
+CR_API cr_status cr_beams_upload(cr_context* ctx, const cr_photon_beam* beams, uint32_t n,
+                                 const cr_photon_beam_data* data, uint32_t n_data);
+CR_API cr_status cr_beams_count(cr_context* ctx, uint32_t* n, uint32_t* n_data);
+CR_API cr_status cr_beams_download(cr_context* ctx, cr_photon_beam* beams, cr_photon_beam_data* data);
+
+/* LBVH build over the device beams: 30-bit Morton codes of centroids in the
+ * cloud box, 4 x 8-bit stable LSD radix sort, Karras hierarchy, AABB refit.
+ * Beams are reordered on the device into sorted order (as the reference's
+ * sort moves whole PhotonBeam records). */
+CR_API cr_status cr_lbvh_build(cr_context* ctx);
+/* Host copies of the last build: sorted codes (n), permutation (n: sorted
+ * position -> input index), nodes (2n-1, reference TreeNode layout).  Any
+ * pointer may be NULL. */
+CR_API cr_status cr_lbvh_download(cr_context* ctx, uint32_t* morton_sorted, uint32_t* perm, cr_tree_node* nodes);
+/* Per-stage device times of the last build in ms: morton, sort, hierarchy, refit, pack. */
+CR_API cr_status cr_lbvh_timings(cr_context* ctx, double out_ms[5]);
+
+/* Gather options (extension). */
+typedef struct cr_gather_ext {
+    uint32_t refCompat; /* 1: PPB_PE.comp:362-404 term as written (sin^2, no 1/r, unscaled density, segment-relative T_beam) */
+    uint32_t jitter;    /* 1: sub-pixel jitter drawn from the pixel's RNG stream */
+    float albedo;       /* scattering albedo used in the corrected term */
+    uint32_t _reserved;
+} cr_gather_ext;
+CR_API void cr_gather_ext_default(cr_gather_ext* g);
+/* PPB_PE.comp main (:471-487) per frame with running-mean accumulation into
+ * the same RGBA32F image as cr_pt_render.  Uses the current shadow volume
+ * as camera transmittance (main.cpp:226-229 builds it along the camera). */
+CR_API cr_status cr_ppb_gather(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames,
+                               const cr_gather_ext* ext);
+typedef struct cr_gather_stats {
+    uint64_t rays, nodesVisited, beamsTested;
+    double lastKernelMs;
+} cr_gather_stats;
+CR_API cr_status cr_gather_stats_get(cr_context* ctx, cr_gather_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CR_ABI_H */

@@ -1,0 +1,114 @@
+/*
+ * ORACLE — test infrastructure only.  Nothing in the product links this.
+ *
+ * Elementary functions of the oracle, written from the mathematics so that
+ * the CPU oracle and the gfx950 kernels can evaluate the same sequence of
+ * IEEE-754 binary32 operations and agree bit for bit.  The reference calls
+ * GLSL log/exp/sin/cos/pow, whose accuracy the Vulkan driver defines
+ * (shaders/PathTracer.comp:249, :289-290, :307-308, :338;
+ * shaders/ShadowVolume.comp:92); any implementation within a few ULP is a
+ * faithful restatement.  Each function below is pinned against double
+ * precision libm in tests/test_oracle_math.py.
+ *
+ *  o_logf   : x = m * 2^e with m in [sqrt(1/2), sqrt(2)], s = (m-1)/(m+1),
+ *             log m = 2 atanh s = 2s (1 + s^2/3 + s^4/5 + ... + s^10/11)
+ *             (the atanh series used by Cody & Waite, "Software Manual for
+ *             the Elementary Functions", 1980), log x = e ln2 + log m with
+ *             ln2 split in a high and a low part.
+ *  o_expf   : x = k ln2 + r, |r| <= ln2/2 (Cody-Waite reduction), Taylor
+ *             series of e^r to r^8, scaled by 2^k.
+ *  o_sincosf: quadrant reduction by pi/2 in three parts (Cody-Waite),
+ *             Taylor series on [-pi/4, pi/4] (sin to r^11, cos to r^12).
+ *  o_pow15  : x^1.5 = x * sqrt(x) (sqrt is correctly rounded in IEEE-754).
+ *
+ * Compile with -ffp-contract=off and without -ffast-math.
+ */
+#ifndef CR_ORACLE_OMATH_H
+#define CR_ORACLE_OMATH_H
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+static inline uint32_t o_f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float o_u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+static inline float o_logf(float x)
+{
+    if (x == 0.0f) return -INFINITY;
+    if (!(x > 0.0f)) return NAN;
+    if (x == INFINITY) return INFINITY;
+    int32_t e = 0;
+    uint32_t b = o_f2u(x);
+    if (b < 0x00800000u) {            /* subnormal: scale by 2^23 (exact) */
+        x = x * 8388608.0f;
+        b = o_f2u(x);
+        e = -23;
+    }
+    e += (int32_t)((b >> 23) & 0xffu) - 127;
+    float m = o_u2f((b & 0x007fffffu) | 0x3f800000u); /* [1,2) */
+    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float z = s * s;
+    float p = 0.0909090909f;          /* 1/11 */
+    p = p * z + 0.111111111f;         /* 1/9  */
+    p = p * z + 0.142857143f;         /* 1/7  */
+    p = p * z + 0.2f;                 /* 1/5  */
+    p = p * z + 0.333333333f;         /* 1/3  */
+    p = p * z;
+    float s2 = s + s;
+    float logm = s2 + s2 * p;
+    float fe = (float)e;
+    return fe * 0.693145751953125f + (logm + fe * 1.42860682030941723212e-6f);
+}
+
+static inline float o_expf(float x)
+{
+    if (x != x) return x;
+    if (x > 88.7f) return INFINITY;
+    if (x < -87.0f) return 0.0f;      /* results below 2^-126 are returned as 0 */
+    float k = floorf(x * 1.44269504f + 0.5f);
+    float r = (x - k * 0.693145751953125f) - k * 1.42860682030941723212e-6f;
+    float p = 2.48015873e-5f;         /* 1/8! */
+    p = p * r + 1.98412698e-4f;       /* 1/7! */
+    p = p * r + 1.38888889e-3f;       /* 1/6! */
+    p = p * r + 8.33333333e-3f;       /* 1/5! */
+    p = p * r + 4.16666667e-2f;       /* 1/4! */
+    p = p * r + 0.166666667f;         /* 1/3! */
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    int32_t ki = (int32_t)k;          /* in [-126, 128] */
+    if (ki > 127) return p * 2.0f * o_u2f((uint32_t)(127 + 127) << 23);
+    if (ki < -126) return p * o_u2f((uint32_t)(ki + 127 + 64) << 23) * 5.42101086e-20f; /* 2^-64 */
+    return p * o_u2f((uint32_t)(ki + 127) << 23);
+}
+
+static inline void o_sincosf(float x, float* s_out, float* c_out)
+{
+    float k = floorf(x * 0.636619772f + 0.5f);           /* round(x / (pi/2)) */
+    float r = ((x - k * 1.5703125f) - k * 4.83751296997070312e-4f) - k * 7.54978995489188216e-8f;
+    float z = r * r;
+    float ps = -2.50521084e-8f;       /* -1/11! */
+    ps = ps * z + 2.75573192e-6f;     /*  1/9!  */
+    ps = ps * z - 1.98412698e-4f;     /* -1/7!  */
+    ps = ps * z + 8.33333333e-3f;     /*  1/5!  */
+    ps = ps * z - 0.166666667f;       /* -1/3!  */
+    float sn = r + r * (z * ps);
+    float pc = 2.08767570e-9f;        /*  1/12! */
+    pc = pc * z - 2.75573192e-7f;     /* -1/10! */
+    pc = pc * z + 2.48015873e-5f;     /*  1/8!  */
+    pc = pc * z - 1.38888889e-3f;     /* -1/6!  */
+    pc = pc * z + 4.16666667e-2f;     /*  1/4!  */
+    pc = pc * z - 0.5f;
+    float cs = 1.0f + z * pc;
+    int32_t q = ((int32_t)k) & 3;
+    if (q == 0) { *s_out = sn;  *c_out = cs; }
+    else if (q == 1) { *s_out = cs;  *c_out = -sn; }
+    else if (q == 2) { *s_out = -sn; *c_out = -cs; }
+    else { *s_out = -cs; *c_out = sn; }
+}
+
+static inline float o_pow15(float x) { return x * sqrtf(x); }
+
+#endif /* CR_ORACLE_OMATH_H */

@@ -1,0 +1,142 @@
+"""GPU parity of the photon-beam pipeline: beam producer, LBVH build
+(Morton codes, sorted permutation, hierarchy, refit) and gather, bit-exact
+against the oracle (oracle/oracle_lbvh.cpp, oracle/oracle_ppb.cpp)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpu_util import assert_bitexact, oracle_scene_from_ctx
+
+pytestmark = pytest.mark.gpu
+
+
+def random_beams(S, n, seed, dup_frac=0.0):
+    rng = np.random.default_rng(seed)
+    beams = (S.PhotonBeam * n)()
+    a = np.frombuffer(beams, dtype=np.uint8).reshape(n, 48)
+    st = rng.uniform(-0.45, 0.45, (n, 3)).astype(np.float32)
+    st[:, 2] += 0.5
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    en = (st + d * rng.uniform(0.0, 0.004, (n, 1))).astype(np.float32)
+    if dup_frac > 0:
+        k = int(n * dup_frac)
+        st[:k] = st[0]
+        en[:k] = en[0]
+    f = a.view(np.float32)
+    u = a.view(np.uint32)
+    f[:, 0:3] = st
+    f[:, 4:7] = en
+    f[:, 7] = 0.002
+    u[:, 8] = rng.integers(0, 4, n)
+    u[:, 3] = 0
+    data = (S.PhotonBeamData * 4)()
+    for i in range(4):
+        data[i].power[:] = (1.0, 1.0, 1.0, 1.0)
+        data[i].trDistances[:] = tuple(np.linspace(0.001, 0.05, 16) * (i + 1))
+    return beams, data
+
+
+def nodes_array(nodes):
+    return np.frombuffer(nodes, dtype=np.uint8).copy()
+
+
+def setup_c1(ctx, cfg, S, size=48):
+    sc = cfg.c1(width=48, height=32, size=size, sv_axis=40)
+    cloud, svp = sc.setup(ctx, build_sv=False)
+    return sc, cloud, svp
+
+
+@pytest.mark.parametrize("n,dup", [(1, 0.0), (2, 0.0), (3, 0.0), (1000, 0.0), (5000, 0.3), (70001, 0.0)])
+def test_lbvh_bitexact_random(ctx, cfg, S, O, n, dup):
+    sc, cloud, _ = setup_c1(ctx, cfg, S)
+    beams, data = random_beams(S, n, seed=n, dup_frac=dup)
+    ctx.upload_beams(beams, data)
+    ctx.lbvh_build()
+    keys, perm, nodes = ctx.lbvh_download()
+    rk, rp, rn = O.lbvh_build(beams, cloud.bounds[0], cloud.bounds[1])
+    np.testing.assert_array_equal(keys, rk)
+    np.testing.assert_array_equal(perm, rp)
+    assert (np.frombuffer(nodes, np.uint8) == np.frombuffer(rn, np.uint8)).all(), "tree nodes differ"
+    # unique stable order: by key, ties by index
+    codes = np.empty(n, np.uint32)
+    O.lib.or_beam_morton30(beams, n, (C.c_float * 3)(*cloud.bounds[0][:3]), (C.c_float * 3)(*cloud.bounds[1][:3]),
+                           codes.ctypes.data)
+    np.testing.assert_array_equal(perm, np.argsort(codes, kind="stable"))
+
+
+def test_trace_beams_bitexact_and_lbvh(ctx, cfg, S, O):
+    sc, cloud, svp = setup_c1(ctx, cfg, S)
+    p = cfg.beam_params(300, seed=38, radius=0.004, albedo=0.9)
+    nb, nd = ctx.trace_beams(p, 1 << 20)
+    gb, gd = ctx.download_beams()
+    s, _, _ = oracle_scene_from_ctx(O, ctx, sc.cam, sc.params, svp, sc.ext, with_sv=False)
+    rb, rd, rnb, rnd = O.trace_beams(s, p, 1 << 20)
+    assert (nb, nd) == (rnb, rnd) and nb > 300
+    assert bytes(gb) == bytes(rb)[: nb * 48], "beams differ"
+    assert bytes(gd) == bytes(rd)[: nd * 80], "beam data differ"
+    # truncation keeps the first max_beams segments in path order
+    nb2, nd2 = ctx.trace_beams(p, nb // 3)
+    gb2, gd2 = ctx.download_beams()
+    assert nb2 == nb // 3 and bytes(gb2) == bytes(gb)[: nb2 * 48]
+    # LBVH over the traced beams
+    ctx.trace_beams(p, 1 << 20)
+    ctx.lbvh_build()
+    keys, perm, nodes = ctx.lbvh_download()
+    rk, rp, rn = O.lbvh_build(gb, cloud.bounds[0], cloud.bounds[1])
+    np.testing.assert_array_equal(keys, rk)
+    np.testing.assert_array_equal(perm, rp)
+    assert bytes(nodes) == bytes(rn)
+
+
+@pytest.mark.parametrize("ref_compat,jitter", [(0, 0), (1, 0), (0, 1)])
+def test_gather_bitexact(ctx, cfg, S, O, ref_compat, jitter):
+    sc, cloud, _ = setup_c1(ctx, cfg, S)
+    # camera transmittance: shadow volume along the camera forward (main.cpp:226-229)
+    svp = cfg.shadow_volume(cloud, tuple(sc.cam.forward), 40)
+    ctx.set_shadow_volume(svp)
+    ctx.sv_build()
+    p = cfg.beam_params(400, seed=11, radius=0.01, albedo=0.9)
+    nb, nd = ctx.trace_beams(p, 1 << 20)
+    ctx.lbvh_build()
+    ext = S.GatherExt(ref_compat, jitter, 0.9, 0)
+    ctx.clear_image()
+    fr = cfg.frames(2)
+    ctx.gather(fr, ext)
+    gpu = ctx.download_image(sc.height, sc.width)
+    st = ctx.gather_stats()
+    gb, gd = ctx.download_beams()
+    keys, perm, nodes = ctx.lbvh_download()
+    sorted_beams = (S.PhotonBeam * nb)(*[gb[int(i)] for i in perm])
+    for i in range(nb):
+        sorted_beams[i].mortonCode = int(keys[i])
+    s, _, _ = oracle_scene_from_ctx(O, ctx, sc.cam, sc.params, svp, sc.ext)
+    b = O.beam_scene(s, sorted_beams, gd, nodes, nb, ref_compat, jitter, 0.9)
+    ref = O.gather_render(b, fr)
+    assert_bitexact(gpu, ref, f"gather ref_compat={ref_compat} jitter={jitter}")
+    assert gpu[..., 0].max() > 0.0
+    assert st.rays >= sc.width * sc.height * 2
+
+
+def test_gather_bvh_equals_brute_force(ctx, cfg, S, O):
+    """The BVH sum equals the loop over every beam (up to reassociation)."""
+    sc, cloud, _ = setup_c1(ctx, cfg, S)
+    svp = cfg.shadow_volume(cloud, tuple(sc.cam.forward), 40)
+    ctx.set_shadow_volume(svp)
+    ctx.sv_build()
+    p = cfg.beam_params(150, seed=3, radius=0.02, albedo=0.9)
+    nb, nd = ctx.trace_beams(p, 1 << 20)
+    ctx.lbvh_build()
+    ctx.clear_image()
+    fr = cfg.frames(1)
+    ctx.gather(fr, S.GatherExt(0, 0, 0.9, 0))
+    gpu = ctx.download_image(sc.height, sc.width)
+    gb, gd = ctx.download_beams()
+    keys, perm, nodes = ctx.lbvh_download()
+    sorted_beams = (S.PhotonBeam * nb)(*[gb[int(i)] for i in perm])
+    s, _, _ = oracle_scene_from_ctx(O, ctx, sc.cam, sc.params, svp, sc.ext)
+    b = O.beam_scene(s, sorted_beams, gd, nodes, nb, 0, 0, 0.9)
+    for (x, y) in [(24, 16), (10, 20), (30, 8), (20, 25)]:
+        brute = O.gather_pixel(b, x, y, fr[0].seed, brute=True)
+        np.testing.assert_allclose(gpu[y, x], brute, rtol=2e-4, atol=1e-6)
