@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the volumetric hot path on one MI355X.
+
+Default workload (the production-size plain path tracer): a 1024x512x1024
+FP16 density grid (1 GiB, fBm anvil, 8 octaves) in 4^3-voxel bricks with
+16^3 brick majorants, ratio-tracked shadow rays, albedo 0.999, HG g = 0.85,
+1920x1080, up to 256 bounces.  One step = one persistent launch that renders
+`--spp-per-step` frames (1 spp each) of the full image; inputs are resident
+in HBM before timing starts.
+
+Prints one JSON line: value = path vertices per second (camera-ray origins
+plus real collisions) over the timed steps, with
+  roofline     : algorithmic bytes of the path-tracing kernel per launch
+                 (16 B per FP16 trilinear lookup, 4 B per brick-majorant
+                 visit, 16 B per shadow-volume lookup, 32 B per pixel) over
+                 its average launch time, measured with HIP events on the
+                 context stream, against 8 TB/s;
+  cpu_baseline : the CPU oracle on the host cores, on a bounded sample of the
+                 same render (a centred pixel rectangle, same streams).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time to relMSE≤1e-3 at 1080p (learned vs plain PT); Mpath-vertices/s; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--spp-per-step", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pt_c2_traffic.json"))
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the path would shard by image tiles with one sum-reduce of the RGBA32F image)", file=sys.stderr)
+        sys.exit(2)
+    import cr_import
+    pkg = cr_import.package()
+    cfg = cr_import.configs()
+    S = pkg.structs
+
+    sc = cfg.c2() if args.workload == "c2" else cfg.c1()
+    ctx = pkg.Context(0)
+    t0 = time.time()
+    cloud, svp = sc.setup(ctx)
+    log(f"{sc.name}: grid {tuple(sc.fbm.size)} generated, shadow volume {ctx.sv_dims()} built in "
+        f"{time.time() - t0:.1f}s")
+    spp = args.spp_per_step
+    total_frames = (args.warmup + args.steps) * spp
+    frames = cfg.frames(total_frames)
+    W, H = sc.width, sc.height
+
+    for w in range(args.warmup):
+        ctx.pt_render(frames[w * spp:(w + 1) * spp])
+    ctx.synchronize()
+    log(f"warmup {args.warmup} step(s) done")
+    ctx.pt_stats_reset()
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        i = (args.warmup + k) * spp
+        ctx.pt_render(frames[i:i + spp])
+    ctx.synchronize()
+    t_total = time.perf_counter() - t_start
+    st = ctx.pt_stats()
+    log(f"timed {args.steps} step(s): {t_total:.3f}s wall, kernel {st.totalKernelMs:.1f} ms over {st.launches} launches")
+
+    vertices = st.vertices
+    value = vertices / t_total / 1e6
+    kern_ms = st.totalKernelMs / max(st.launches, 1)
+    bytes_per_launch = (16.0 * st.lookups + 4.0 * st.brickVisits + 16.0 * st.svLookups) / max(st.launches, 1) \
+        + 32.0 * W * H
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        try:
+            tf = json.load(open(args.traffic_file))
+            if tf.get("workload") == sc.name and tf.get("spp_per_step") == spp:
+                traffic = tf.get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ctx, sc, svp, cfg, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpath-vertices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{sc.name}: plain path tracer, {sc.fbm.size[0]}x{sc.fbm.size[1]}x{sc.fbm.size[2]} "
+                         f"{'FP16' if sc.storage == S.CR_DTYPE_F16 else 'FP32'} fBm grid, {W}x{H}, "
+                         f"{spp} spp per step, <= {sc.ext.bounceCap} bounces, albedo {sc.ext.albedo:.3f}, "
+                         f"g {sc.params.phaseG:.2f}, "
+                         f"{'ratio-tracked shadow rays' if sc.ext.neeMode == S.CR_NEE_RATIO_TRACKING else 'shadow-volume NEE'}, "
+                         f"{'brick-majorant DDA' if sc.ext.trackingMode == S.CR_TRACK_BRICK_DDA else 'global majorant'}"),
+            "spp_per_step": spp,
+            "image": [W, H],
+            "grid": list(sc.fbm.size),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "pt_kernel",
+            "kernel_ms": round(kern_ms, 3),
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
+        },
+        "cpu_baseline": cpu,
+        "extra": {
+            "paths": st.paths,
+            "vertices": st.vertices,
+            "lookups": st.lookups,
+            "shadow_volume_lookups": st.svLookups,
+            "brick_visits": st.brickVisits,
+            "Mlookups_per_s": round(st.lookups / t_total / 1e6, 2),
+            "Mpaths_per_s": round(st.paths / t_total / 1e6, 3),
+            "spp_per_s_1080p": round(st.paths / t_total / (W * H), 3),
+        },
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(ctx, sc, svp, cfg, seconds):
+    """The oracle on the host cores over a centred pixel rectangle of the
+    same render (same pixel ids and RNG streams), sized to about `seconds`."""
+    import numpy as np
+    import cr_import
+    O = cr_import.oracle()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    grid = ctx.download_grid()
+    if sc.storage == 1:
+        grid = grid.astype(np.float16)
+    s = O.scene(grid, ctx.cloud(), sc.cam, sc.params, svp, sc.ext, sv=ctx.sv_download(), majorants=ctx.majorants())
+    fr = cfg.frames(1)
+    W, H = sc.width, sc.height
+    # calibrate on a small block, then time a block sized to the target
+    w = 32
+    h = 8
+    t = time.perf_counter()
+    _, cnt = O.pt_render_region(s, fr, W // 2 - w // 2, H // 2 - h // 2, w, h, threads)
+    dt = time.perf_counter() - t
+    per_px = dt / (w * h)
+    npx = int(max(256, min(W * H, seconds / max(per_px, 1e-9))))
+    h = max(1, min(H, npx // W))
+    w = W if h > 1 else min(W, npx)
+    x0, y0 = (W - w) // 2, (H - h) // 2
+    t = time.perf_counter()
+    _, cnt = O.pt_render_region(s, fr, x0, y0, w, h, threads)
+    dt = time.perf_counter() - t
+    log(f"cpu baseline: {w}x{h} px x 1 spp in {dt:.1f}s on {threads} threads")
+    return {
+        "value": round(cnt.vertices / dt / 1e6, 4),
+        "unit": "Mpath-vertices/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle (oracle/liboracle.so, C++ -O3, OpenMP) rendering the centred {w}x{h} pixels of the same "
+                  f"{W}x{H} image, 1 spp, same pixel ids and RNG streams; {cnt.vertices} vertices in {dt:.1f}s",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -72,6 +72,16 @@ SIGNATURES = {
     "cr_gather_ext_default": (None, [P(S.GatherExt)]),
     "cr_ppb_gather": (st, [ctx_p, P(S.PushConstants), C.c_uint32, P(S.GatherExt)]),
     "cr_gather_stats_get": (st, [ctx_p, P(S.GatherStats)]),
+    "cr_mlp_param_layout": (None, [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]),
+    "cr_mlp_init": (st, [ctx_p, C.c_uint32]),
+    "cr_mlp_set_params": (st, [ctx_p, vp, C.c_uint32]),
+    "cr_mlp_get_params": (st, [ctx_p, vp, C.c_uint32]),
+    "cr_mlp_forward": (st, [ctx_p, vp, C.c_uint32, vp]),
+    "cr_mlp_train_step": (st, [ctx_p, vp, vp, C.c_uint32, C.c_float, P(C.c_float), vp]),
+    "cr_mlp_probe": (st, [ctx_p, vp, vp, vp]),
+    "cr_math_probe": (st, [ctx_p, C.c_int, vp, C.c_uint32, vp]),
+    "cr_learn_build_mips": (st, [ctx_p]),
+    "cr_learn_features": (st, [ctx_p, vp, C.c_uint32, vp]),
 }
 
 

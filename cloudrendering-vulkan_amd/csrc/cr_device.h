@@ -34,10 +34,29 @@ __device__ __forceinline__ f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, 
 __device__ __forceinline__ f3 mul(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
 __device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ float length(f3 a) { return __fsqrt_rn(dot(a, a)); }
+// Correctly rounded binary32 square root.  The hardware square root
+// (v_sqrt_f32, which __fsqrt_rn lowers to) is accurate to one ULP only, so its
+// result s is corrected against the midpoints between s and its two
+// neighbours: a midpoint has 25 significant bits and its square 50, so the
+// products and comparisons below are exact in binary64, and no binary32 input
+// can equal such a square (no ties).
+__device__ __forceinline__ float fsqrt(float a)
+{
+    const float s = __fsqrt_rn(a);
+    if (!(a > 0.0f) || a == __builtin_inff()) return s;
+    const double ad = a;
+    const float lo = __uint_as_float(__float_as_uint(s) - 1u);
+    const double mlo = 0.5 * ((double)lo + (double)s);
+    if (ad < mlo * mlo) return lo;
+    const float hi = __uint_as_float(__float_as_uint(s) + 1u);
+    const double mhi = 0.5 * ((double)s + (double)hi);
+    if (ad > mhi * mhi) return hi;
+    return s;
+}
+__device__ __forceinline__ float length(f3 a) { return fsqrt(dot(a, a)); }
 __device__ __forceinline__ f3 normalize(f3 a)
 {
-    float l = __fsqrt_rn(dot(a, a));
+    float l = fsqrt(dot(a, a));
     return mk(__fdiv_rn(a.x, l), __fdiv_rn(a.y, l), __fdiv_rn(a.z, l));
 }
 __device__ __forceinline__ f3 cross(f3 a, f3 b)
@@ -47,7 +66,6 @@ __device__ __forceinline__ f3 cross(f3 a, f3 b)
 __device__ __forceinline__ f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
 
 __device__ __forceinline__ float fdiv(float a, float b) { return __fdiv_rn(a, b); }
-__device__ __forceinline__ float fsqrt(float a) { return __fsqrt_rn(a); }
 
 __device__ __forceinline__ float crd_logf(float x)
 {

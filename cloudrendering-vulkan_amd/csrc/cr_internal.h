@@ -77,6 +77,7 @@ hipError_t launch_grid_to_f32(const void* src, int dtype, int layout, float* dst
                               int32_t nbx, int32_t nby, hipStream_t s);
 hipError_t launch_majorants(const void* grid, int dtype, int layout, int32_t nx, int32_t ny, int32_t nz, int32_t nbx,
                             int32_t nby, float* out, int32_t mx, int32_t my, int32_t mz, hipStream_t s);
+hipError_t launch_math_probe(int op, const float* in, uint32_t n, float* out, hipStream_t s);
 hipError_t launch_grid_max(const float* majorants, uint32_t n, unsigned int* out_bits, hipStream_t s);
 
 } // namespace cr

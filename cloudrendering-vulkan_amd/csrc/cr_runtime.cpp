@@ -139,6 +139,12 @@ cr_status cr_destroy(cr_context* ctx)
     for (void* b : bufs)
         if (b) hipFree(b);
     ppb_free(ctx);
+    mlp_free(ctx);
+    learn_free(ctx);
+    for (auto& ev : ctx->ptEvents) {
+        hipEventDestroy(ev.first);
+        hipEventDestroy(ev.second);
+    }
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->gev0) hipEventDestroy(ctx->gev0);
@@ -335,6 +341,23 @@ cr_status cr_grid_majorants(cr_context* ctx, uint32_t dims[3], float* out)
         CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     }
     return CR_OK;
+}
+
+cr_status cr_math_probe(cr_context* ctx, int op, const float* in, uint32_t n, float* out)
+{
+    CR_CHECK_CTX(ctx);
+    if (op < CR_MATH_SQRT || op > CR_MATH_COS) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "unknown math op");
+    if (n == 0) return CR_OK;
+    if (!in || !out) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null array");
+    float* d = nullptr;
+    hipError_t e = hipMalloc(&d, (size_t)n * 8);
+    if (e != hipSuccess) return hip_fail(ctx, e, "probe buffer");
+    e = hipMemcpyAsync(d, in, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_math_probe(op, d, n, d + n, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + n, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    return e == hipSuccess ? CR_OK : hip_fail(ctx, e, "math probe");
 }
 
 // Grid3D<float>::Load (Grid3D.h:44-90): 3 x u32 counts, 3 x f64 voxel sizes,
@@ -552,6 +575,22 @@ cr_status cr_pt_clear(cr_context* ctx)
 
 namespace cr {
 
+// Wait for the recorded render launches and add their durations.
+cr_status fold_pt_events(cr_context* ctx)
+{
+    if (ctx->ptEventsUsed == 0) return CR_OK;
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    for (size_t i = 0; i < ctx->ptEventsUsed; i++) {
+        float ms = 0.0f;
+        CR_HIP(ctx, hipEventElapsedTime(&ms, ctx->ptEvents[i].first, ctx->ptEvents[i].second), "hipEventElapsedTime");
+        ctx->ptTotalMs += ms;
+        ctx->ptLaunches++;
+        ctx->ptLastMs = ms;
+    }
+    ctx->ptEventsUsed = 0;
+    return CR_OK;
+}
+
 cr_status upload_frames(cr_context* ctx, const cr_push_constants* frames, uint32_t n)
 {
     std::vector<int32_t> fr(2 * (size_t)n);
@@ -619,10 +658,21 @@ cr_status cr_pt_render(cr_context* ctx, const cr_push_constants* frames, uint32_
     if ((uint32_t)blocks > needed) blocks = (int)needed;
     if (ctx->ptBlocksOverride > 0) blocks = ctx->ptBlocksOverride;
     CR_HIP(ctx, zero_async(ctx->dQueue, 4, ctx->stream), "zero queue");
-    CR_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream), "event");
+    if (ctx->ptEventsUsed >= 1024) {
+        st = fold_pt_events(ctx);
+        if (st != CR_OK) return st;
+    }
+    if (ctx->ptEventsUsed == ctx->ptEvents.size()) {
+        std::pair<hipEvent_t, hipEvent_t> ev;
+        CR_HIP(ctx, hipEventCreate(&ev.first), "hipEventCreate");
+        CR_HIP(ctx, hipEventCreate(&ev.second), "hipEventCreate");
+        ctx->ptEvents.push_back(ev);
+    }
+    const auto& ev = ctx->ptEvents[ctx->ptEventsUsed];
+    CR_HIP(ctx, hipEventRecord(ev.first, ctx->stream), "event");
     CR_HIP(ctx, launch_pt(a, g.dtype, g.layout, blocks, ctx->stream), "path tracer kernel");
-    CR_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream), "event");
-    ctx->ptPending = true;
+    CR_HIP(ctx, hipEventRecord(ev.second, ctx->stream), "event");
+    ctx->ptEventsUsed++;
     return CR_OK;
 }
 
@@ -658,19 +708,21 @@ cr_status cr_pt_stats_get(cr_context* ctx, cr_pt_stats* out)
     out->lookups = c[2];
     out->svLookups = c[3];
     out->brickVisits = c[4];
-    float ms = 0.0f;
-    if (ctx->ptPending) {
-        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-        ctx->ptLastMs = ms;
-        ctx->ptPending = false;
-    }
+    cr_status st = fold_pt_events(ctx);
+    if (st != CR_OK) return st;
     out->lastKernelMs = ctx->ptLastMs;
+    out->totalKernelMs = ctx->ptTotalMs;
+    out->launches = ctx->ptLaunches;
     return CR_OK;
 }
 
 cr_status cr_pt_stats_reset(cr_context* ctx)
 {
     CR_CHECK_CTX(ctx);
+    cr_status st = fold_pt_events(ctx);
+    if (st != CR_OK) return st;
+    ctx->ptTotalMs = 0.0;
+    ctx->ptLaunches = 0;
     CR_HIP(ctx, zero_async(ctx->dCounters, 8 * sizeof(unsigned long long), ctx->stream), "zero counters");
     return CR_OK;
 }

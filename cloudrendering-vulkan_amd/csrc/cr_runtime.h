@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "cr_internal.h"
 
@@ -54,10 +56,18 @@ struct cr_context {
     size_t scalarCap = 0;
     bool ptPending = false;
     float ptLastMs = 0.0f;
+    // per-launch event pairs, folded into ptTotalMs when stats are read
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ptEvents;
+    size_t ptEventsUsed = 0;
+    double ptTotalMs = 0.0;
+    uint64_t ptLaunches = 0;
     int ptBlocksOverride = 0;
 
     // photon beams / LBVH (cr_runtime_ppb.cpp)
     struct Ppb* ppb = nullptr;
+    // learned in-scattering network (cr_runtime_mlp.cpp) and its support data (cr_runtime_learn.cpp)
+    struct Mlp* mlp = nullptr;
+    struct Learn* learn = nullptr;
 };
 
 namespace cr {
@@ -67,4 +77,8 @@ cr_status ensure_buffer(cr_context* ctx, void** p, size_t* cap, size_t bytes, co
 hipError_t zero_async(void* p, size_t bytes, hipStream_t s);
 cr_status upload_frames(cr_context* ctx, const cr_push_constants* frames, uint32_t n);
 void ppb_free(cr_context* ctx);
+void mlp_free(cr_context* ctx);
+void learn_free(cr_context* ctx);
+cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr);
+cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y);
 } // namespace cr

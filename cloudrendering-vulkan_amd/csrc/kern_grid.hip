@@ -248,6 +248,26 @@ hipError_t launch_majorants(const void* grid, int dtype, int layout, int32_t nx,
     return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256) math_probe_kernel(int op, const float* in, uint32_t n, float* out)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float x = in[i];
+    float r = 0.0f, sn = 0.0f, cs = 0.0f;
+    if (op == CR_MATH_SQRT) r = fsqrt(x);
+    else if (op == CR_MATH_LOG) r = crd_logf(x);
+    else if (op == CR_MATH_EXP) r = crd_expf(x);
+    else { crd_sincosf(x, &sn, &cs); r = op == CR_MATH_SIN ? sn : cs; }
+    out[i] = r;
+}
+
+hipError_t launch_math_probe(int op, const float* in, uint32_t n, float* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(math_probe_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, op, in, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_grid_max(const float* majorants, uint32_t n, unsigned int* out_bits, hipStream_t s)
 {
     unsigned blocks = (unsigned)((n + 255) / 256);

@@ -25,6 +25,9 @@ using namespace crd;
 namespace {
 
 constexpr uint32_t NONE = 0xFFFFFFFFu;
+// Ceiling on delta-tracking steps of one walk along a beam (a walk through the
+// unit box with majorant 64 takes ~64 steps); it only bounds the loop.
+constexpr uint32_t TRACK_CEILING = 1u << 16;
 
 template <typename T, int LAYOUT>
 __device__ __forceinline__ float cloud_at(const void* grid, int32_t nx, int32_t ny, int32_t nz, int32_t nbx, int32_t nby,
@@ -113,10 +116,10 @@ __global__ void __launch_bounds__(256) trace_kernel(cr::TraceArgs a)
 #pragma unroll 1
             for (int s = 0; s < 16; s++) {
                 float t = 0.0f;
-                for (;;) {
+                for (uint32_t step = 0; step < TRACK_CEILING; step++) {
                     const float zeta = xs_float(rng);
                     t = t + fdiv(-crd_logf(zeta), sigmaBar);
-                    if (t > len) break;
+                    if (!(t <= len)) break;
                     const f3 cur = add(pos, mul(dir, t));
                     const float e = cloud_at<T, LAYOUT>(a.grid, a.nx, a.ny, a.nz, a.nbx, a.nby, cp, cur);
                     const float xi = xs_float(rng);
@@ -156,15 +159,15 @@ __global__ void __launch_bounds__(256) trace_kernel(cr::TraceArgs a)
             // next real collision — PPB_PT.comp:473-494
             float t = 0.0f;
             f3 cur = pos;
-            bool left = false;
-            for (;;) {
+            bool left = true;
+            for (uint32_t step = 0; step < TRACK_CEILING; step++) {
                 const float zeta = xs_float(rng);
                 t = t + fdiv(-crd_logf(zeta), sigmaBar);
-                if (t > len) { left = true; break; }
+                if (!(t <= len)) { left = true; break; }
                 cur = add(pos, mul(dir, t));
                 const float e = cloud_at<T, LAYOUT>(a.grid, a.nx, a.ny, a.nz, a.nbx, a.nby, cp, cur);
                 const float xi = xs_float(rng);
-                if (xi < fdiv(e, cp.maxExtinction)) break;
+                if (xi < fdiv(e, cp.maxExtinction)) { left = false; break; }
             }
             if (left) break;
             pos = cur;

@@ -174,7 +174,7 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
             }
             L.tau = L.tau - sb * seg;
             L.t = tc;
-            if (tc >= L.dist) return EV_EXIT;
+            if (!(tc < L.dist)) return EV_EXIT;
             if (!dda_advance(a, dir, L)) return EV_EXIT;
         }
         f3 cur = add(L.pos, mul(dir, L.t));
@@ -196,7 +196,7 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
     }
     float zeta = xs_float(L.rng);
     L.t = L.t + fdiv(-crd_logf(zeta), a.sigmaBar);
-    if (L.t >= L.dist) return EV_EXIT;
+    if (!(L.t < L.dist)) return EV_EXIT;
     f3 cur = add(L.pos, mul(dir, L.t));
     float e = sample_cloud<T, LAYOUT>(a, cur);
     c.lookups++;

@@ -71,7 +71,7 @@ class RenderExt(C.Structure):
 
 class PtStats(C.Structure):
     _fields_ = [("paths", u64), ("vertices", u64), ("lookups", u64), ("svLookups", u64), ("brickVisits", u64),
-                ("lastKernelMs", f64)]
+                ("lastKernelMs", f64), ("totalKernelMs", f64), ("launches", u64)]
 
 
 class BeamTraceParams(C.Structure):

@@ -203,6 +203,8 @@ typedef struct cr_pt_stats {
     uint64_t svLookups;  /* shadow-volume lookups */
     uint64_t brickVisits;/* brick-majorant cells entered (DDA tracking) */
     double lastKernelMs; /* duration of the last render launch (HIP events) */
+    double totalKernelMs;/* sum of render-launch durations since the last reset */
+    uint64_t launches;   /* render launches since the last reset */
 } cr_pt_stats;
 CR_API cr_status cr_pt_stats_get(cr_context* ctx, cr_pt_stats* out);
 CR_API cr_status cr_pt_stats_reset(cr_context* ctx);
@@ -269,6 +271,47 @@ typedef struct cr_gather_stats {
     double lastKernelMs;
 } cr_gather_stats;
 CR_API cr_status cr_gather_stats_get(cr_context* ctx, cr_gather_stats* out);
+
+/* --------------------------------------------- learned in-scattering (extension)
+ * A 224 -> 128 -> 128 -> 128 -> 128 -> 3 ReLU network on bf16 MFMA with fp32
+ * accumulation, predicting log(1 + Y) per RGB channel, where Y is the radiance
+ * the plain path tracer adds after a real collision (no reference
+ * counterpart).  Parameters live on the device as fp32 masters in a padded
+ * flat layout (layer l: weights [K_l][N_l] row-major at wOffset[l], bias
+ * [N_l] at bOffset[l]; K = 224,128,128,128,128; N = 128,128,128,128,16 of
+ * which the last layer uses 3) plus bf16 copies used by the kernels. */
+#define CR_MLP_INPUTS 224
+CR_API void cr_mlp_param_layout(uint32_t K[5], uint32_t N[5], uint32_t wOffset[5], uint32_t bOffset[5],
+                                uint32_t* total);
+/* He-normal weights, zero biases, zero Adam state (deterministic in seed). */
+CR_API cr_status cr_mlp_init(cr_context* ctx, uint32_t seed);
+CR_API cr_status cr_mlp_set_params(cr_context* ctx, const float* params, uint32_t n);
+CR_API cr_status cr_mlp_get_params(cr_context* ctx, float* params, uint32_t n);
+/* Forward pass of B samples: X is [B][224] bf16 (host), Y is [B][3] (host). */
+CR_API cr_status cr_mlp_forward(cr_context* ctx, const uint16_t* X, uint32_t B, float* Y);
+/* One Adam step (lr, beta1 0.9, beta2 0.999, eps 1e-8) on the mean squared
+ * error of the output against log(1 + targets); returns the loss and, when
+ * grads is not NULL, the parameter gradients (padded layout) of this step. */
+CR_API cr_status cr_mlp_train_step(cr_context* ctx, const uint16_t* X, const float* targets, uint32_t B, float lr,
+                                   float* loss, float* grads);
+/* One 16x16x16 bf16 MFMA on row-major A (16x16) and B (16x16): D = A B. */
+CR_API cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t* B, float* D);
+
+/* The kernels' elementary functions applied to n floats on the device
+ * (in and out are host arrays).  The path tracer's log/exp/sin/cos/sqrt are
+ * these exact functions; the probe lets a test compare them bit for bit with a
+ * CPU implementation.  No reference counterpart: the reference calls GLSL's
+ * built-ins, whose precision the Vulkan driver defines. */
+enum { CR_MATH_SQRT = 0, CR_MATH_LOG = 1, CR_MATH_EXP = 2, CR_MATH_SIN = 3, CR_MATH_COS = 4 };
+CR_API cr_status cr_math_probe(cr_context* ctx, int op, const float* in, uint32_t n, float* out);
+
+/* Density mip pyramid (8 levels, 2x2x2 means) of the current grid, used by the
+ * network's stencil features; rebuilt after every grid change. */
+CR_API cr_status cr_learn_build_mips(cr_context* ctx);
+/* Stencil features of n shading points: points is [n][8] floats (x, y, z,
+ * omega_x, omega_y, omega_z, unused, unused); X receives [n][224] bf16.  Uses
+ * the current cloud properties and shadow-volume light direction. */
+CR_API cr_status cr_learn_features(cr_context* ctx, const float* points, uint32_t n, uint16_t* X);
 
 #ifdef __cplusplus
 }

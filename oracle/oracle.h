@@ -49,6 +49,8 @@ typedef struct or_counters {
 float or_logf(float x);
 float or_expf(float x);
 void or_sincosf(float x, float* s, float* c);
+/* op is a CR_MATH_* code; sqrt is the C library's (correctly rounded). */
+void or_math_array(int op, const float* in, uint32_t n, float* out);
 
 /* ------------------------------------------------------------------- RNG
  * shaders/PathTracer.comp:220-241 (identical copies PPB_PT.comp:210-231). */
@@ -87,6 +89,9 @@ void or_pt_sample(const or_scene* s, int32_t x, int32_t y, int32_t seed, float o
  * (RGBA32F, W x H); OpenMP over rows when threads > 1. */
 void or_pt_render(const or_scene* s, const cr_push_constants* frames, uint32_t n_frames,
                   float* accum, int32_t threads, or_counters* c);
+/* The same for the pixel rectangle [x0, x0+w) x [y0, y0+h) only. */
+void or_pt_render_region(const or_scene* s, const cr_push_constants* frames, uint32_t n_frames, float* accum,
+                         int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t threads, or_counters* c);
 
 /* ------------------------------------------------------------------ grids */
 uint32_t or_hash32(uint32_t x);

@@ -47,6 +47,8 @@ namespace {
 
 const float INV_4PI = 1.0f / (4.0f * PI_F);
 const uint32_t NONE = 0xFFFFFFFFu;
+/* ceiling on delta-tracking steps of one walk along a beam (same in the kernel) */
+const uint32_t TRACK_CEILING = 1u << 16;
 
 inline float phase(const cr_parameters& p, f3 a, f3 b)
 {
@@ -107,10 +109,10 @@ inline float track_free_path(const or_scene& s, const Grid& g, f3 pos, f3 dir, f
                              uint32_t& rng)
 {
     float t = 0.0f;
-    for (;;) {
+    for (uint32_t step = 0; step < TRACK_CEILING; step++) {
         float zeta = xs_float(rng);
         t = t + (-o_logf(zeta)) / sigmaBar;
-        if (t > len) break;
+        if (!(t <= len)) break;
         f3 cur = add(pos, mul(dir, t));
         float e = sample_cloud(s, g, cur);
         float xi = xs_float(rng);
@@ -202,15 +204,15 @@ PathOut trace_path(const or_scene& s, const cr_beam_trace_params& P, uint32_t id
         float dist = len;
         float t = 0.0f;
         f3 cur = pos;
-        bool left = false;
-        for (;;) {
+        bool left = true;
+        for (uint32_t step = 0; step < TRACK_CEILING; step++) {
             float zeta = xs_float(rng);
             t = t + (-o_logf(zeta)) / sigmaBar;
-            if (t > dist) { left = true; break; }
+            if (!(t <= dist)) { left = true; break; }
             cur = add(pos, mul(dir, t));
             float e = sample_cloud(s, g, cur);
             float xi = xs_float(rng);
-            if (xi < e / s.cloud.maxExtinction) break;
+            if (xi < e / s.cloud.maxExtinction) { left = false; break; }
         }
         if (left) break;
         pos = cur;

@@ -25,6 +25,19 @@ extern "C" {
 float or_logf(float x) { return o_logf(x); }
 float or_expf(float x) { return o_expf(x); }
 void or_sincosf(float x, float* s, float* c) { o_sincosf(x, s, c); }
+void or_math_array(int op, const float* in, uint32_t n, float* out)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        const float x = in[i];
+        float sn, cs;
+        switch (op) {
+        case CR_MATH_SQRT: out[i] = sqrtf(x); break;
+        case CR_MATH_LOG: out[i] = o_logf(x); break;
+        case CR_MATH_EXP: out[i] = o_expf(x); break;
+        default: o_sincosf(x, &sn, &cs); out[i] = op == CR_MATH_SIN ? sn : cs; break;
+        }
+    }
+}
 
 uint32_t or_xorshift32(uint32_t s) { return xs_next(s); }
 
@@ -264,7 +277,7 @@ inline bool find_scatter(const Ctx& C, f3& pos, f3 dir, uint32_t& rng)
             }
             tau = tau - sb * seg;
             t = tc;
-            if (tc >= dist) return false;
+            if (!(tc < dist)) return false;
             if (!dda_advance(C, d)) return false;
         }
     }
@@ -273,7 +286,7 @@ inline bool find_scatter(const Ctx& C, f3& pos, f3 dir, uint32_t& rng)
     for (;;) {
         float zeta = xs_float(rng);
         t = t + (-o_logf(zeta)) / C.sigmaBar;
-        if (t >= dist) return false;
+        if (!(t < dist)) return false;
         cur = add(pos, mul(dir, t));
         float e = sample_cloud(C, cur);
         C.cnt->lookups++;
@@ -316,7 +329,7 @@ inline float ratio_track(const Ctx& C, f3 pos, f3 dir, uint32_t& rng)
             }
             tau = tau - sb * seg;
             t = tc;
-            if (tc >= dist) return T;
+            if (!(tc < dist)) return T;
             if (!dda_advance(C, d)) return T;
         }
     }
@@ -324,7 +337,7 @@ inline float ratio_track(const Ctx& C, f3 pos, f3 dir, uint32_t& rng)
     for (;;) {
         float zeta = xs_float(rng);
         t = t + (-o_logf(zeta)) / C.sigmaBar;
-        if (t >= dist) return T;
+        if (!(t < dist)) return T;
         f3 cur = add(pos, mul(dir, t));
         float e = sample_cloud(C, cur);
         C.cnt->lookups++;
@@ -515,6 +528,41 @@ void or_pt_sample(const or_scene* s, int32_t x, int32_t y, int32_t seed, float o
     init_ctx(C, s, c ? c : &local);
     f4 r = pt_sample(C, x, y, seed);
     out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
+/* The pixels [x0, x0+w) x [y0, y0+h) of the full image (same ids, same
+ * streams); accum is w x h RGBA.  Used to time a bounded sample of a large
+ * render on the host. */
+void or_pt_render_region(const or_scene* s, const cr_push_constants* frames, uint32_t n_frames, float* accum,
+                         int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t threads, or_counters* c)
+{
+    uint64_t tp = 0, tv = 0, tl = 0, ts = 0, tb = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : tp, tv, tl, ts, tb)
+    for (int32_t j = 0; j < h * w; j++) {
+        or_counters cnt = {0, 0, 0, 0, 0};
+        Ctx C;
+        init_ctx(C, s, &cnt);
+        int32_t x = x0 + j % w, y = y0 + j / w;
+        float* a = accum + 4 * (uint64_t)j;
+        for (uint32_t f = 0; f < n_frames; f++) {
+            f4 L = pt_sample(C, x, y, frames[f].seed);
+            float n = (float)frames[f].frameCount;
+            float d = (float)(frames[f].frameCount + 1u);
+            a[0] = (L.x + a[0] * n) / d;
+            a[1] = (L.y + a[1] * n) / d;
+            a[2] = (L.z + a[2] * n) / d;
+            a[3] = (L.w + a[3] * n) / d;
+        }
+        tp += cnt.paths; tv += cnt.vertices; tl += cnt.lookups; ts += cnt.svLookups; tb += cnt.brickVisits;
+    }
+    if (c) {
+        c->paths += tp; c->vertices += tv; c->lookups += tl; c->svLookups += ts; c->brickVisits += tb;
+    }
 }
 
 void or_pt_render(const or_scene* s, const cr_push_constants* frames, uint32_t n_frames, float* accum,

@@ -48,6 +48,7 @@ class Oracle:
             "or_logf": (C.c_float, [C.c_float]),
             "or_expf": (C.c_float, [C.c_float]),
             "or_sincosf": (None, [C.c_float, P(C.c_float), P(C.c_float)]),
+            "or_math_array": (None, [C.c_int, vp, C.c_uint32, vp]),
             "or_xorshift32": (C.c_uint32, [C.c_uint32]),
             "or_rng_stream": (None, [C.c_int32, C.c_uint32, C.c_uint32, P(C.c_uint32)]),
             "or_msvc_rand": (None, [C.c_uint32, C.c_uint32, P(C.c_int32)]),
@@ -62,6 +63,8 @@ class Oracle:
             "or_sv_build": (None, [P(OrScene), vp]),
             "or_pt_sample": (None, [P(OrScene), C.c_int32, C.c_int32, C.c_int32, P(C.c_float), P(OrCounters)]),
             "or_pt_render": (None, [P(OrScene), P(S.PushConstants), C.c_uint32, vp, C.c_int32, P(OrCounters)]),
+            "or_pt_render_region": (None, [P(OrScene), P(S.PushConstants), C.c_uint32, vp, C.c_int32, C.c_int32,
+                                           C.c_int32, C.c_int32, C.c_int32, P(OrCounters)]),
             "or_hash32": (C.c_uint32, [C.c_uint32]),
             "or_fbm_voxel": (C.c_float, [P(S.FbmParams), C.c_uint32, C.c_uint32, C.c_uint32]),
             "or_fbm_grid": (None, [P(S.FbmParams), vp, C.c_int32]),
@@ -151,6 +154,14 @@ class Oracle:
         self.lib.or_pt_render(C.byref(s), arr, len(frames), a.ctypes.data, threads, C.byref(cnt))
         return a, cnt
 
+    def pt_render_region(self, s, frames, x0, y0, w, h, threads=0):
+        a = np.zeros((h, w, 4), np.float32)
+        arr = (self.S.PushConstants * len(frames))(*frames)
+        cnt = self.OrCounters()
+        self.lib.or_pt_render_region(C.byref(s), arr, len(frames), a.ctypes.data, x0, y0, w, h, threads,
+                                     C.byref(cnt))
+        return a, cnt
+
     def pt_sample(self, s, x, y, seed):
         out = (C.c_float * 4)()
         cnt = self.OrCounters()
@@ -158,6 +169,12 @@ class Oracle:
         return np.array(out, np.float32), cnt
 
     # --------------------------------------------------------------- grids
+    def math_array(self, op, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty_like(x)
+        self.lib.or_math_array(op, x.ctypes.data, x.size, out.ctypes.data)
+        return out
+
     def fbm_grid(self, p, threads=0):
         out = np.empty((p.size[2], p.size[1], p.size[0]), np.float32)
         self.lib.or_fbm_grid(C.byref(p), out.ctypes.data, threads)

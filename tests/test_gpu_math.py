@@ -1,0 +1,69 @@
+"""The kernels' elementary functions (sqrt, log, exp, sin, cos) on the device
+against the oracle's CPU implementation, bit for bit, over every binade of the
+inputs the path tracer and the grid generator feed them.
+
+sqrt is also checked against NumPy's float32 square root, which IEEE-754
+requires to be correctly rounded: the hardware square root alone is accurate to
+one ULP, so this is what pins the device's rounding correction.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import assert_bitexact
+
+pytestmark = pytest.mark.gpu
+
+SQRT, LOG, EXP, SIN, COS = 0, 1, 2, 3, 4
+
+
+def probe(pkg, ctx, op, x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    ctx.check(pkg.load().cr_math_probe(ctx.h, op, x.ctypes.data, x.size, out.ctypes.data), "math probe")
+    return out
+
+
+def assert_same(got, ref, what):
+    """Bit-exact, except that any NaN matches any NaN (the payload is not specified)."""
+    nan_g, nan_r = np.isnan(got), np.isnan(ref)
+    assert np.array_equal(nan_g, nan_r), f"{what}: NaN at different inputs"
+    assert_bitexact(got[~nan_g], ref[~nan_r], what)
+
+
+def positive_floats(rng, n):
+    """Uniform over the bit patterns of positive finite floats (denormals included)."""
+    bits = rng.integers(1, 0x7F800000, size=n, dtype=np.uint32)
+    return bits.view(np.float32)
+
+
+def specials():
+    return np.array([0.0, -0.0, 1.0, 2.0, 4.0, 0.25, np.inf, -np.inf, np.nan, -1.0, 1e-45, 1.17549435e-38,
+                     3.4028235e38, 0.5, 0.999999940, 1.00000012], np.float32)
+
+
+def test_sqrt_correctly_rounded(pkg, ctx):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([positive_floats(rng, 1 << 20), rng.random(1 << 18, dtype=np.float32), specials()])
+    got = probe(pkg, ctx, SQRT, x)
+    with np.errstate(invalid="ignore"):
+        ref = np.sqrt(x)
+    assert_same(got, ref, "device sqrt vs IEEE sqrt")
+
+
+@pytest.mark.parametrize("op", [LOG, EXP, SIN, COS])
+def test_elementary_functions_bitexact(pkg, ctx, O, op):
+    rng = np.random.default_rng(10 + op)
+    if op == LOG:
+        x = np.concatenate([positive_floats(rng, 1 << 20), rng.random(1 << 18, dtype=np.float32)])
+    elif op == EXP:
+        x = rng.uniform(-104.0, 89.0, 1 << 20).astype(np.float32)
+    else:
+        x = np.concatenate([rng.uniform(-7.0, 7.0, 1 << 20), rng.uniform(-200.0, 200.0, 1 << 16)]).astype(np.float32)
+    x = np.concatenate([x, specials()])
+    assert_same(probe(pkg, ctx, op, x), O.math_array(op, x), f"device vs oracle, op {op}")
+
+
+def test_probe_rejects_unknown_op(pkg, ctx):
+    x = np.ones(4, np.float32)
+    out = np.empty_like(x)
+    assert pkg.load().cr_math_probe(ctx.h, 9, x.ctypes.data, 4, out.ctypes.data) != 0

@@ -38,8 +38,19 @@ def random_beams(S, n, seed, dup_frac=0.0):
     return beams, data
 
 
-def nodes_array(nodes):
-    return np.frombuffer(nodes, dtype=np.uint8).copy()
+def assert_same_tree(gpu_nodes, ref_nodes, n):
+    a = np.frombuffer(gpu_nodes, np.uint8).reshape(-1, 64)[: 2 * n - 1]
+    b = np.frombuffer(ref_nodes, np.uint8).reshape(-1, 64)[: 2 * n - 1]
+    if (a == b).all():
+        return
+    names = ["parent", "left", "right", "isLeaf", "lo.x", "lo.y", "lo.z", "lo.w", "hi.x", "hi.y", "hi.z", "hi.w",
+             "processed", "p0", "p1", "p2"]
+    au, bu = a.view(np.uint32), b.view(np.uint32)
+    bad_nodes = np.nonzero((au != bu).any(axis=1))[0]
+    i = int(bad_nodes[0])
+    fields = [names[k] for k in range(16) if au[i, k] != bu[i, k]]
+    raise AssertionError(f"{len(bad_nodes)} of {2 * n - 1} nodes differ; first node {i} (inner={n - 1}) fields "
+                         f"{fields}: gpu={au[i].tolist()} oracle={bu[i].tolist()}")
 
 
 def setup_c1(ctx, cfg, S, size=48):
@@ -56,9 +67,13 @@ def test_lbvh_bitexact_random(ctx, cfg, S, O, n, dup):
     ctx.lbvh_build()
     keys, perm, nodes = ctx.lbvh_download()
     rk, rp, rn = O.lbvh_build(beams, cloud.bounds[0], cloud.bounds[1])
-    np.testing.assert_array_equal(keys, rk)
-    np.testing.assert_array_equal(perm, rp)
-    assert (np.frombuffer(nodes, np.uint8) == np.frombuffer(rn, np.uint8)).all(), "tree nodes differ"
+    codes = np.empty(n, np.uint32)
+    O.lib.or_beam_morton30(beams, n, (C.c_float * 3)(*cloud.bounds[0][:3]), (C.c_float * 3)(*cloud.bounds[1][:3]),
+                           codes.ctypes.data)
+    assert np.array_equal(np.sort(keys), np.sort(codes)), "Morton codes differ (as multisets)"
+    np.testing.assert_array_equal(keys, rk, err_msg="sorted keys")
+    np.testing.assert_array_equal(perm, rp, err_msg="permutation")
+    assert_same_tree(nodes, rn, n)
     # unique stable order: by key, ties by index
     codes = np.empty(n, np.uint32)
     O.lib.or_beam_morton30(beams, n, (C.c_float * 3)(*cloud.bounds[0][:3]), (C.c_float * 3)(*cloud.bounds[1][:3]),
@@ -68,12 +83,12 @@ def test_lbvh_bitexact_random(ctx, cfg, S, O, n, dup):
 
 def test_trace_beams_bitexact_and_lbvh(ctx, cfg, S, O):
     sc, cloud, svp = setup_c1(ctx, cfg, S)
-    p = cfg.beam_params(300, seed=38, radius=0.004, albedo=0.9)
+    p = cfg.beam_params(100, seed=38, radius=0.004, albedo=0.9, max_interactions=16)
     nb, nd = ctx.trace_beams(p, 1 << 20)
     gb, gd = ctx.download_beams()
     s, _, _ = oracle_scene_from_ctx(O, ctx, sc.cam, sc.params, svp, sc.ext, with_sv=False)
     rb, rd, rnb, rnd = O.trace_beams(s, p, 1 << 20)
-    assert (nb, nd) == (rnb, rnd) and nb > 300
+    assert (nb, nd) == (rnb, rnd) and nb > 100
     assert bytes(gb) == bytes(rb)[: nb * 48], "beams differ"
     assert bytes(gd) == bytes(rd)[: nd * 80], "beam data differ"
     # truncation keeps the first max_beams segments in path order
@@ -87,7 +102,7 @@ def test_trace_beams_bitexact_and_lbvh(ctx, cfg, S, O):
     rk, rp, rn = O.lbvh_build(gb, cloud.bounds[0], cloud.bounds[1])
     np.testing.assert_array_equal(keys, rk)
     np.testing.assert_array_equal(perm, rp)
-    assert bytes(nodes) == bytes(rn)
+    assert_same_tree(nodes, rn, nb)
 
 
 @pytest.mark.parametrize("ref_compat,jitter", [(0, 0), (1, 0), (0, 1)])
@@ -97,7 +112,7 @@ def test_gather_bitexact(ctx, cfg, S, O, ref_compat, jitter):
     svp = cfg.shadow_volume(cloud, tuple(sc.cam.forward), 40)
     ctx.set_shadow_volume(svp)
     ctx.sv_build()
-    p = cfg.beam_params(400, seed=11, radius=0.01, albedo=0.9)
+    p = cfg.beam_params(200, seed=11, radius=0.01, albedo=0.9, max_interactions=16)
     nb, nd = ctx.trace_beams(p, 1 << 20)
     ctx.lbvh_build()
     ext = S.GatherExt(ref_compat, jitter, 0.9, 0)
@@ -125,7 +140,7 @@ def test_gather_bvh_equals_brute_force(ctx, cfg, S, O):
     svp = cfg.shadow_volume(cloud, tuple(sc.cam.forward), 40)
     ctx.set_shadow_volume(svp)
     ctx.sv_build()
-    p = cfg.beam_params(150, seed=3, radius=0.02, albedo=0.9)
+    p = cfg.beam_params(150, seed=3, radius=0.02, albedo=0.9, max_interactions=16)
     nb, nd = ctx.trace_beams(p, 1 << 20)
     ctx.lbvh_build()
     ctx.clear_image()
