@@ -4,9 +4,10 @@
 Default workload (the production-size plain path tracer): a 1024x512x1024
 FP16 density grid (1 GiB, fBm anvil, 8 octaves) in 4^3-voxel bricks with
 16^3 brick majorants, ratio-tracked shadow rays, albedo 0.999, HG g = 0.85,
-1920x1080, up to 256 bounces.  One step = one persistent launch that renders
-`--spp-per-step` frames (1 spp each) of the full image; inputs are resident
-in HBM before timing starts.
+1920x1080, up to 256 bounces, 256 spp.  One step = one persistent launch
+that renders the workload's frames (1 spp each; `--spp-per-step` overrides
+the count) of the full image; inputs are resident in HBM before timing
+starts.
 
 Prints one JSON line: value = path vertices per second (camera-ray origins
 plus real collisions) over the timed steps, with
@@ -35,10 +36,10 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--spp-per-step", type=int, default=4)
+    ap.add_argument("--spp-per-step", type=int, default=0, help="frames per step; 0 = the workload's full spp")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pt_c2_traffic.json"))
@@ -66,7 +67,7 @@ def main():
     cloud, svp = sc.setup(ctx)
     log(f"{sc.name}: grid {tuple(sc.fbm.size)} generated, shadow volume {ctx.sv_dims()} built in "
         f"{time.time() - t0:.1f}s")
-    spp = args.spp_per_step
+    spp = args.spp_per_step or sc.spp
     total_frames = (args.warmup + args.steps) * spp
     frames = cfg.frames(total_frames)
     W, H = sc.width, sc.height
@@ -167,30 +168,31 @@ def cpu_baseline(ctx, sc, svp, cfg, seconds):
     if sc.storage == 1:
         grid = grid.astype(np.float16)
     s = O.scene(grid, ctx.cloud(), sc.cam, sc.params, svp, sc.ext, sv=ctx.sv_download(), majorants=ctx.majorants())
-    fr = cfg.frames(1)
     W, H = sc.width, sc.height
-    # calibrate on a small block, then time a block sized to the target
-    w = 32
-    h = 8
+    # calibrate on a centred block, then time a block (and frame count) sized to the target
+    w, h = min(W, 64), min(H, 32)
     t = time.perf_counter()
-    _, cnt = O.pt_render_region(s, fr, W // 2 - w // 2, H // 2 - h // 2, w, h, threads)
+    _, cnt = O.pt_render_region(s, cfg.frames(1), W // 2 - w // 2, H // 2 - h // 2, w, h, threads)
     dt = time.perf_counter() - t
     per_px = dt / (w * h)
-    npx = int(max(256, min(W * H, seconds / max(per_px, 1e-9))))
+    npx = int(max(256, seconds / max(per_px, 1e-9)))
+    nfr = int(max(1, min(sc.spp, npx // (W * H))))
+    npx = min(npx, W * H)
     h = max(1, min(H, npx // W))
     w = W if h > 1 else min(W, npx)
     x0, y0 = (W - w) // 2, (H - h) // 2
+    fr = cfg.frames(nfr)
     t = time.perf_counter()
     _, cnt = O.pt_render_region(s, fr, x0, y0, w, h, threads)
     dt = time.perf_counter() - t
-    log(f"cpu baseline: {w}x{h} px x 1 spp in {dt:.1f}s on {threads} threads")
+    log(f"cpu baseline: {w}x{h} px x {nfr} spp in {dt:.1f}s on {threads} threads")
     return {
         "value": round(cnt.vertices / dt / 1e6, 4),
         "unit": "Mpath-vertices/s",
         "cores": threads,
         "kind": "port",
         "sample": f"oracle (oracle/liboracle.so, C++ -O3, OpenMP) rendering the centred {w}x{h} pixels of the same "
-                  f"{W}x{H} image, 1 spp, same pixel ids and RNG streams; {cnt.vertices} vertices in {dt:.1f}s",
+                  f"{W}x{H} image, {nfr} spp, same pixel ids and RNG streams; {cnt.vertices} vertices in {dt:.1f}s",
     }
 
 

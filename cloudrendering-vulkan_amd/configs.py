@@ -74,7 +74,7 @@ def shadow_volume(cloud, light_dir=(1.0, -1.0, 0.0), axis_count=256):
 
 
 def render_ext(albedo=1.0, bounce_cap=S.CR_UNBOUNDED, sky=None, nee=S.CR_NEE_SHADOW_VOLUME,
-               tracking=S.CR_TRACK_GLOBAL_MAJORANT, sv_dims=(0, 0, 0)):
+               tracking=S.CR_TRACK_GLOBAL_MAJORANT, sv_dims=(0, 0, 0), majorant_brick=0):
     e = S.RenderExt()
     load().cr_render_ext_default(C.byref(e))
     e.albedo = albedo
@@ -85,6 +85,7 @@ def render_ext(albedo=1.0, bounce_cap=S.CR_UNBOUNDED, sky=None, nee=S.CR_NEE_SHA
     e.neeMode = nee
     e.trackingMode = tracking
     e.svDims[:] = sv_dims
+    e.majorantBrick = majorant_brick
     return e
 
 
@@ -169,7 +170,7 @@ def c2(width=1920, height=1080, spp=256, size=(1024, 512, 1024), storage=S.CR_DT
     vs = 1.0 / size[0]
     svd = (256, 128, 256) if tuple(size) == (1024, 512, 1024) else (max(2, size[0] // 4), max(2, size[1] // 4), max(2, size[2] // 4))
     return Scene("c2", fbm(tuple(size), vs, seed=7, octaves=8, shape=S.CR_SHAPE_ANVIL, lo=0.16, hi=0.36),
-                 storage, layout, 1.0, 64.0, camera(width, height), parameters(0.85),
+                 storage, layout, 1.0, 64.0, camera(width, height, position=(0.0, 0.0, -0.15)), parameters(0.85),
                  render_ext(albedo=0.999, bounce_cap=bounce_cap, sky=SKY, nee=nee, tracking=tracking, sv_dims=svd),
                  256, spp)
 

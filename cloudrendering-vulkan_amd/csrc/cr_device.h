@@ -13,7 +13,7 @@
 //                 log x = e ln2 + log m with ln2 = hi + lo.
 //   crd_expf    : Cody-Waite reduction x = k ln2 + r, Taylor series to r^8.
 //   crd_sincosf : reduction by pi/2 in three parts, Taylor series of sin to
-//                 r^11 and cos to r^12 on [-pi/4, pi/4].
+//                 r^11 and cos to r^12 on [-pi/4, pi/4]; NaN for |x| > 2^17.
 //   x^1.5       : x * sqrt(x) (sqrt correctly rounded).
 #pragma once
 
@@ -120,6 +120,11 @@ __device__ __forceinline__ float crd_expf(float x)
 
 __device__ __forceinline__ void crd_sincosf(float x, float* s_out, float* c_out)
 {
+    if (!(fabsf(x) <= 131072.0f)) {
+        *s_out = __builtin_nanf("");
+        *c_out = __builtin_nanf("");
+        return;
+    }
     float k = floorf(x * 0.636619772f + 0.5f);
     float r = ((x - k * 1.5703125f) - k * 4.83751296997070312e-4f) - k * 7.54978995489188216e-8f;
     float z = r * r;

@@ -17,8 +17,9 @@ struct DeviceGrid {
     int32_t nx = 0, ny = 0, nz = 0;
     int32_t nbx = 0, nby = 0, nbz = 0;
     size_t bytes = 0;
-    float* majorants = nullptr; // ceil(N/16)^3 brick maxima (with apron)
+    float* majorants = nullptr; // ceil(N/mb)^3 brick maxima (with apron)
     int32_t mx = 0, my = 0, mz = 0;
+    int32_t mb = 16;            // voxels per side of a majorant brick
 };
 
 struct PTArgs {
@@ -76,7 +77,7 @@ hipError_t launch_convert_grid(const void* src, int src_dtype, int src_layout_zf
 hipError_t launch_grid_to_f32(const void* src, int dtype, int layout, float* dst, int32_t nx, int32_t ny, int32_t nz,
                               int32_t nbx, int32_t nby, hipStream_t s);
 hipError_t launch_majorants(const void* grid, int dtype, int layout, int32_t nx, int32_t ny, int32_t nz, int32_t nbx,
-                            int32_t nby, float* out, int32_t mx, int32_t my, int32_t mz, hipStream_t s);
+                            int32_t nby, int32_t mb, float* out, int32_t mx, int32_t my, int32_t mz, hipStream_t s);
 hipError_t launch_math_probe(int op, const float* in, uint32_t n, float* out, hipStream_t s);
 hipError_t launch_grid_max(const float* majorants, uint32_t n, unsigned int* out_bits, hipStream_t s);
 

@@ -207,7 +207,15 @@ cr_status grid_alloc(cr_context* ctx, int32_t nx, int32_t ny, int32_t nz, int dt
     g.layout = layout;
     g.nx = nx; g.ny = ny; g.nz = nz;
     g.nbx = nbx; g.nby = nby; g.nbz = nbz;
-    int32_t mx = (nx + 15) / 16, my = (ny + 15) / 16, mz = (nz + 15) / 16;
+    return CR_OK;
+}
+
+int majorant_brick(const cr_render_ext& e) { return e.majorantBrick ? (int)e.majorantBrick : 16; }
+
+// (Re)build the brick-majorant grid of g for mb^3 bricks.
+cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb)
+{
+    int32_t mx = (g.nx + mb - 1) / mb, my = (g.ny + mb - 1) / mb, mz = (g.nz + mb - 1) / mb;
     if (g.majorants && (g.mx != mx || g.my != my || g.mz != mz)) {
         hipFree(g.majorants);
         g.majorants = nullptr;
@@ -219,7 +227,9 @@ cr_status grid_alloc(cr_context* ctx, int32_t nx, int32_t ny, int32_t nz, int dt
             return fail(ctx, CR_ERR_OUT_OF_MEMORY, "hipMalloc majorants");
         }
     }
-    g.mx = mx; g.my = my; g.mz = mz;
+    g.mx = mx; g.my = my; g.mz = mz; g.mb = mb;
+    CR_HIP(ctx, launch_majorants(g.data, g.dtype, g.layout, g.nx, g.ny, g.nz, g.nbx, g.nby, mb, g.majorants, mx, my, mz,
+                                 ctx->stream), "majorant kernel");
     return CR_OK;
 }
 
@@ -227,8 +237,8 @@ cr_status grid_alloc(cr_context* ctx, int32_t nx, int32_t ny, int32_t nz, int dt
 cr_status grid_finish(cr_context* ctx, double vx, double vy, double vz)
 {
     DeviceGrid& g = ctx->grid;
-    CR_HIP(ctx, launch_majorants(g.data, g.dtype, g.layout, g.nx, g.ny, g.nz, g.nbx, g.nby, g.majorants, g.mx, g.my, g.mz,
-                                 ctx->stream), "majorant kernel");
+    cr_status st = majorants_build(ctx, g, majorant_brick(ctx->ext));
+    if (st != CR_OK) return st;
     CR_HIP(ctx, zero_async(ctx->dScalar, 4, ctx->stream), "zero");
     CR_HIP(ctx, launch_grid_max(g.majorants, (uint32_t)((size_t)g.mx * g.my * g.mz), (unsigned int*)ctx->dScalar, ctx->stream),
            "max kernel");
@@ -495,8 +505,13 @@ cr_status cr_set_render_ext(cr_context* ctx, const cr_render_ext* e)
     if (anyDim && (!allDim || e->svDims[0] < 2 || e->svDims[1] < 2 || e->svDims[2] < 2 || e->svDims[0] > 4096 ||
                    e->svDims[1] > 4096 || e->svDims[2] > 4096))
         return fail(ctx, CR_ERR_INVALID_ARGUMENT, "svDims must be all zero or all in [2, 4096]");
+    if (e->majorantBrick != 0 && e->majorantBrick != 4 && e->majorantBrick != 8 && e->majorantBrick != 16 &&
+        e->majorantBrick != 32)
+        return fail(ctx, CR_ERR_INVALID_ARGUMENT, "majorantBrick must be 0, 4, 8, 16 or 32");
     if (memcmp(e->svDims, ctx->ext.svDims, sizeof(e->svDims)) != 0) ctx->svValid = false;
+    const int mb = majorant_brick(*e);
     ctx->ext = *e;
+    if (ctx->hasGrid && ctx->grid.mb != mb) return majorants_build(ctx, ctx->grid, mb);
     return CR_OK;
 }
 
@@ -611,7 +626,8 @@ cr_status fill_pt_args(cr_context* ctx, PTArgs& a, uint32_t n_frames)
     a.grid = g.data; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nbx = g.nbx; a.nby = g.nby;
     a.majorants = g.majorants; a.mx = g.mx; a.my = g.my; a.mz = g.mz;
     const int32_t dims[3] = {g.nx, g.ny, g.nz};
-    for (int k = 0; k < 3; k++) a.bw[k] = ((ctx->cloud.bounds[1][k] - ctx->cloud.bounds[0][k]) * 16.0f) / (float)dims[k];
+    for (int k = 0; k < 3; k++)
+        a.bw[k] = ((ctx->cloud.bounds[1][k] - ctx->cloud.bounds[0][k]) * (float)g.mb) / (float)dims[k];
     a.sv = ctx->sv;
     a.svx = (int32_t)ctx->svDimsBuilt[0]; a.svy = (int32_t)ctx->svDimsBuilt[1]; a.svz = (int32_t)ctx->svDimsBuilt[2];
     a.cam = ctx->cam;

@@ -76,6 +76,8 @@ cr_status hip_fail(cr_context* ctx, hipError_t e, const char* what);
 cr_status ensure_buffer(cr_context* ctx, void** p, size_t* cap, size_t bytes, const char* what);
 hipError_t zero_async(void* p, size_t bytes, hipStream_t s);
 cr_status upload_frames(cr_context* ctx, const cr_push_constants* frames, uint32_t n);
+int majorant_brick(const cr_render_ext& e);
+cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb);
 void ppb_free(cr_context* ctx);
 void mlp_free(cr_context* ctx);
 void learn_free(cr_context* ctx);

@@ -13,8 +13,8 @@
 //   fbm     : sum_o amp_o noise(p f_o), amp_0 = 1/2, amp *= gain, f *= lacunarity,
 //             normalised by sum_o amp_o.
 //   shape   : cumulus = radial falloff (0.20 to 0.48) about (0.5, 0.42, 0.5) of the normalised
-//             box; anvil = height envelope times a radius widening from 0.12
-//             to 0.42 between heights 0.55 and 0.85.
+//             box; anvil = height envelope times a radius widening from 0.22
+//             to 0.48 between heights 0.50 and 0.80.
 //   density : smoothstep(thresholdLo, thresholdHi, fbm * shape).
 #include "cr_device.h"
 #include "cr_internal.h"
@@ -68,8 +68,8 @@ __device__ __forceinline__ float shape_fn(const cr_fbm_params& p, float ux, floa
     if (p.shape == CR_SHAPE_ANVIL) {
         float dx = ux - 0.5f, dz = uz - 0.5f;
         float rh = fsqrt(dx * dx + dz * dz);
-        float R = 0.12f + 0.30f * smoothstep_f(0.55f, 0.85f, uy);
-        float env = smoothstep_f(0.04f, 0.14f, uy) * (1.0f - smoothstep_f(0.86f, 0.96f, uy));
+        float R = 0.22f + 0.26f * smoothstep_f(0.50f, 0.80f, uy);
+        float env = smoothstep_f(0.02f, 0.10f, uy) * (1.0f - smoothstep_f(0.90f, 0.98f, uy));
         return env * (1.0f - smoothstep_f(R * 0.55f, R, rh));
     }
     float dx = ux - 0.5f, dy = (uy - 0.42f) * 1.25f, dz = uz - 0.5f;
@@ -146,17 +146,17 @@ __global__ void __launch_bounds__(256) to_f32_kernel(const T* src, float* dst, i
         load_voxel<T>(src, vidx<LAYOUT>(x, y, z, nx, ny, nbx, nby));
 }
 
-// One block per 16^3 brick: max over the brick plus a one-voxel apron
-// (the 18^3 voxels a trilinear lookup inside the brick can touch), start 0
-// as Grid3D::GetMajorant starts from T().
+// One block per mb^3 brick: max over the brick plus a one-voxel apron
+// (the (mb+2)^3 voxels a trilinear lookup inside the brick can touch), start
+// 0 as Grid3D::GetMajorant starts from T().
 template <typename T, int LAYOUT>
 __global__ void __launch_bounds__(256) majorant_kernel(const T* g, int32_t nx, int32_t ny, int32_t nz, int32_t nbx,
-                                                       int32_t nby, float* out, int32_t mx, int32_t my)
+                                                       int32_t nby, int32_t mb, float* out, int32_t mx, int32_t my)
 {
     const int32_t bx = (int32_t)blockIdx.x, by = (int32_t)blockIdx.y, bz = (int32_t)blockIdx.z;
-    const int32_t x0 = max(bx * 16 - 1, 0), x1 = min(bx * 16 + 16, nx - 1);
-    const int32_t y0 = max(by * 16 - 1, 0), y1 = min(by * 16 + 16, ny - 1);
-    const int32_t z0 = max(bz * 16 - 1, 0), z1 = min(bz * 16 + 16, nz - 1);
+    const int32_t x0 = max(bx * mb - 1, 0), x1 = min(bx * mb + mb, nx - 1);
+    const int32_t y0 = max(by * mb - 1, 0), y1 = min(by * mb + mb, ny - 1);
+    const int32_t z0 = max(bz * mb - 1, 0), z1 = min(bz * mb + mb, nz - 1);
     const int32_t ex = x1 - x0 + 1, ey = y1 - y0 + 1, ez = z1 - z0 + 1;
     const int32_t total = ex * ey * ez;
     float m = 0.0f;
@@ -235,15 +235,15 @@ hipError_t launch_grid_to_f32(const void* src, int dtype, int layout, float* dst
 }
 
 hipError_t launch_majorants(const void* grid, int dtype, int layout, int32_t nx, int32_t ny, int32_t nz, int32_t nbx,
-                            int32_t nby, float* out, int32_t mx, int32_t my, int32_t mz, hipStream_t s)
+                            int32_t nby, int32_t mb, float* out, int32_t mx, int32_t my, int32_t mz, hipStream_t s)
 {
     dim3 g((unsigned)mx, (unsigned)my, (unsigned)mz);
     if (dtype == CR_DTYPE_F16) {
-        if (layout == LAYOUT_BRICK4) hipLaunchKernelGGL((majorant_kernel<__half, LAYOUT_BRICK4>), g, dim3(256), 0, s, (const __half*)grid, nx, ny, nz, nbx, nby, out, mx, my);
-        else hipLaunchKernelGGL((majorant_kernel<__half, LAYOUT_DENSE>), g, dim3(256), 0, s, (const __half*)grid, nx, ny, nz, nbx, nby, out, mx, my);
+        if (layout == LAYOUT_BRICK4) hipLaunchKernelGGL((majorant_kernel<__half, LAYOUT_BRICK4>), g, dim3(256), 0, s, (const __half*)grid, nx, ny, nz, nbx, nby, mb, out, mx, my);
+        else hipLaunchKernelGGL((majorant_kernel<__half, LAYOUT_DENSE>), g, dim3(256), 0, s, (const __half*)grid, nx, ny, nz, nbx, nby, mb, out, mx, my);
     } else {
-        if (layout == LAYOUT_BRICK4) hipLaunchKernelGGL((majorant_kernel<float, LAYOUT_BRICK4>), g, dim3(256), 0, s, (const float*)grid, nx, ny, nz, nbx, nby, out, mx, my);
-        else hipLaunchKernelGGL((majorant_kernel<float, LAYOUT_DENSE>), g, dim3(256), 0, s, (const float*)grid, nx, ny, nz, nbx, nby, out, mx, my);
+        if (layout == LAYOUT_BRICK4) hipLaunchKernelGGL((majorant_kernel<float, LAYOUT_BRICK4>), g, dim3(256), 0, s, (const float*)grid, nx, ny, nz, nbx, nby, mb, out, mx, my);
+        else hipLaunchKernelGGL((majorant_kernel<float, LAYOUT_DENSE>), g, dim3(256), 0, s, (const float*)grid, nx, ny, nz, nbx, nby, mb, out, mx, my);
     }
     return hipGetLastError();
 }

@@ -66,7 +66,7 @@ class FbmParams(C.Structure):
 
 class RenderExt(C.Structure):
     _fields_ = [("albedo", f32), ("bounceCap", u32), ("skyMode", u32), ("skyColor", f32 * 4), ("neeMode", u32),
-                ("trackingMode", u32), ("svDims", u32 * 3), ("_reserved", u32 * 4)]
+                ("trackingMode", u32), ("svDims", u32 * 3), ("majorantBrick", u32), ("_reserved", u32 * 3)]
 
 
 class PtStats(C.Structure):

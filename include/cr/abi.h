@@ -167,7 +167,8 @@ typedef struct cr_render_ext {
     uint32_t neeMode;      /* CR_NEE_* */
     uint32_t trackingMode; /* CR_TRACK_* */
     uint32_t svDims[3];    /* shadow-volume texels per axis; 0,0,0 = voxelAxisCount^3 */
-    uint32_t _reserved[4];
+    uint32_t majorantBrick; /* voxels per side of a majorant brick: 4, 8, 16 or 32; 0 = 16 */
+    uint32_t _reserved[3];
 } cr_render_ext;
 enum { CR_SKY_GRADIENT = 0 /* PathTracer.comp:104-113 */, CR_SKY_CONSTANT = 1 };
 enum { CR_NEE_SHADOW_VOLUME = 0 /* PathTracer.comp:405-411 */, CR_NEE_RATIO_TRACKING = 1, CR_NEE_NONE = 2 };

@@ -8,7 +8,7 @@
  * (shaders/PathTracer.comp:249, :289-290, :307-308, :338;
  * shaders/ShadowVolume.comp:92); any implementation within a few ULP is a
  * faithful restatement.  Each function below is pinned against double
- * precision libm in tests/test_oracle_math.py.
+ * precision libm in tests/test_oracle.py.
  *
  *  o_logf   : x = m * 2^e with m in [sqrt(1/2), sqrt(2)], s = (m-1)/(m+1),
  *             log m = 2 atanh s = 2s (1 + s^2/3 + s^4/5 + ... + s^10/11)
@@ -18,7 +18,10 @@
  *  o_expf   : x = k ln2 + r, |r| <= ln2/2 (Cody-Waite reduction), Taylor
  *             series of e^r to r^8, scaled by 2^k.
  *  o_sincosf: quadrant reduction by pi/2 in three parts (Cody-Waite),
- *             Taylor series on [-pi/4, pi/4] (sin to r^11, cos to r^12).
+ *             Taylor series on [-pi/4, pi/4] (sin to r^11, cos to r^12);
+ *             defined for |x| <= 2^17, NaN beyond (the three-part reduction
+ *             loses accuracy there, and the quadrant would need a float to
+ *             int conversion that C leaves undefined for huge values).
  *  o_pow15  : x^1.5 = x * sqrt(x) (sqrt is correctly rounded in IEEE-754).
  *
  * Compile with -ffp-contract=off and without -ffast-math.
@@ -86,6 +89,11 @@ static inline float o_expf(float x)
 
 static inline void o_sincosf(float x, float* s_out, float* c_out)
 {
+    if (!(fabsf(x) <= 131072.0f)) {
+        *s_out = NAN;
+        *c_out = NAN;
+        return;
+    }
     float k = floorf(x * 0.636619772f + 0.5f);           /* round(x / (pi/2)) */
     float r = ((x - k * 1.5703125f) - k * 4.83751296997070312e-4f) - k * 7.54978995489188216e-8f;
     float z = r * r;

@@ -101,8 +101,8 @@ uint16_t or_f32_to_f16(float f);
 float or_f16_to_f32(uint16_t h);
 /* Grid3D::GetMajorant (Grid3D.h:180-192). */
 float or_grid_max(const float* grid, uint64_t n);
-/* Brick majorants: max over each 16^3 brick plus a one-voxel apron. */
-void or_brick_majorants(const float* grid, const uint32_t dims[3], float* out, uint32_t mdims_out[3]);
+/* Brick majorants: max over each brick^3 brick plus a one-voxel apron. */
+void or_brick_majorants(const float* grid, const uint32_t dims[3], uint32_t brick, float* out, uint32_t mdims_out[3]);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,7 @@
  *               f_0 = baseFrequency, f *= lacunarity, divided by sum_o amp_o.
  *   shape     : cumulus = radial falloff (0.20 to 0.48) around (0.5, 0.42, 0.5) of the
  *               normalized box; anvil = height envelope times a radius that
- *               widens from 0.12 to 0.42 between heights 0.55 and 0.85.
+ *               widens from 0.22 to 0.48 between heights 0.50 and 0.80.
  *   density   : smoothstep(thresholdLo, thresholdHi, fbm * shape).
  */
 #include <math.h>
@@ -93,8 +93,8 @@ inline float shape_fn(const cr_fbm_params& p, float ux, float uy, float uz)
     if (p.shape == CR_SHAPE_ANVIL) {
         float dx = ux - 0.5f, dz = uz - 0.5f;
         float rh = sqrtf(dx * dx + dz * dz);
-        float R = 0.12f + 0.30f * smoothstep_f(0.55f, 0.85f, uy);
-        float env = smoothstep_f(0.04f, 0.14f, uy) * (1.0f - smoothstep_f(0.86f, 0.96f, uy));
+        float R = 0.22f + 0.26f * smoothstep_f(0.50f, 0.80f, uy);
+        float env = smoothstep_f(0.02f, 0.10f, uy) * (1.0f - smoothstep_f(0.90f, 0.98f, uy));
         return env * (1.0f - smoothstep_f(R * 0.55f, R, rh));
     }
     float dx = ux - 0.5f, dy = (uy - 0.42f) * 1.25f, dz = uz - 0.5f;
@@ -161,19 +161,19 @@ float or_grid_max(const float* grid, uint64_t n)
     return m;
 }
 
-void or_brick_majorants(const float* grid, const uint32_t dims[3], float* out, uint32_t md[3])
+void or_brick_majorants(const float* grid, const uint32_t dims[3], uint32_t brick, float* out, uint32_t md[3])
 {
-    const int64_t nx = dims[0], ny = dims[1], nz = dims[2];
-    md[0] = (uint32_t)((nx + 15) / 16);
-    md[1] = (uint32_t)((ny + 15) / 16);
-    md[2] = (uint32_t)((nz + 15) / 16);
+    const int64_t nx = dims[0], ny = dims[1], nz = dims[2], b = brick;
+    md[0] = (uint32_t)((nx + b - 1) / b);
+    md[1] = (uint32_t)((ny + b - 1) / b);
+    md[2] = (uint32_t)((nz + b - 1) / b);
     for (int64_t bz = 0; bz < md[2]; bz++)
         for (int64_t by = 0; by < md[1]; by++)
             for (int64_t bx = 0; bx < md[0]; bx++) {
                 float m = 0.0f;
-                int64_t x0 = std::max<int64_t>(bx * 16 - 1, 0), x1 = std::min<int64_t>(bx * 16 + 16, nx - 1);
-                int64_t y0 = std::max<int64_t>(by * 16 - 1, 0), y1 = std::min<int64_t>(by * 16 + 16, ny - 1);
-                int64_t z0 = std::max<int64_t>(bz * 16 - 1, 0), z1 = std::min<int64_t>(bz * 16 + 16, nz - 1);
+                int64_t x0 = std::max<int64_t>(bx * b - 1, 0), x1 = std::min<int64_t>(bx * b + b, nx - 1);
+                int64_t y0 = std::max<int64_t>(by * b - 1, 0), y1 = std::min<int64_t>(by * b + b, ny - 1);
+                int64_t z0 = std::max<int64_t>(bz * b - 1, 0), z1 = std::min<int64_t>(bz * b + b, nz - 1);
                 for (int64_t z = z0; z <= z1; z++)
                     for (int64_t y = y0; y <= y1; y++)
                         for (int64_t x = x0; x <= x1; x++) {

@@ -176,7 +176,7 @@ struct Ctx {
     float sigmaBar;   /* global majorant extinction: maxExtinction*densityScaling/baseScaling */
     float extScale;   /* densityScaling / baseScaling is never formed: see findScatterPoint */
     int32_t mx, my, mz;
-    float bw[3];      /* world size of a 16-voxel brick per axis */
+    float bw[3];      /* world size of a majorant brick per axis */
     or_counters* cnt;
 };
 
@@ -371,7 +371,8 @@ void init_ctx(Ctx& C, const or_scene* s, or_counters* cnt)
     C.my = (int32_t)s->mdims[1];
     C.mz = (int32_t)s->mdims[2];
     for (int k = 0; k < 3; k++)
-        C.bw[k] = ((s->cloud.bounds[1][k] - s->cloud.bounds[0][k]) * 16.0f) / (float)s->dims[k];
+        C.bw[k] = ((s->cloud.bounds[1][k] - s->cloud.bounds[0][k]) * (float)(s->ext.majorantBrick ? s->ext.majorantBrick : 16u)) /
+                  (float)s->dims[k];
     C.cnt = cnt;
 }
 

@@ -71,7 +71,7 @@ class Oracle:
             "or_f32_to_f16": (C.c_uint16, [C.c_float]),
             "or_f16_to_f32": (C.c_float, [C.c_uint16]),
             "or_grid_max": (C.c_float, [vp, C.c_uint64]),
-            "or_brick_majorants": (None, [vp, P(C.c_uint32), vp, P(C.c_uint32)]),
+            "or_brick_majorants": (None, [vp, P(C.c_uint32), C.c_uint32, vp, P(C.c_uint32)]),
             "or_expand_bits10": (C.c_uint32, [C.c_uint32]),
             "or_morton24_ref": (C.c_uint32, [P(C.c_float), P(C.c_float)]),
             "or_morton30": (C.c_uint32, [P(C.c_float), P(C.c_float), P(C.c_float)]),
@@ -112,7 +112,7 @@ class Oracle:
         nz, ny, nx = g.shape
         s.dims[:] = (nx, ny, nz)
         if majorants is None:
-            majorants, md = self.brick_majorants(np.asarray(grid, np.float32))
+            majorants, md = self.brick_majorants(np.asarray(grid, np.float32), ext.majorantBrick or 16)
         else:
             md = majorants.shape[::-1]
         majorants = np.ascontiguousarray(majorants, np.float32)
@@ -180,14 +180,15 @@ class Oracle:
         self.lib.or_fbm_grid(C.byref(p), out.ctypes.data, threads)
         return out
 
-    def brick_majorants(self, grid):
+    def brick_majorants(self, grid, brick=16):
         g = np.ascontiguousarray(grid, np.float32)
         nz, ny, nx = g.shape
-        md = ((nx + 15) // 16, (ny + 15) // 16, (nz + 15) // 16)
+        b = brick
+        md = ((nx + b - 1) // b, (ny + b - 1) // b, (nz + b - 1) // b)
         out = np.empty((md[2], md[1], md[0]), np.float32)
         dims = (C.c_uint32 * 3)(nx, ny, nz)
         mo = (C.c_uint32 * 3)()
-        self.lib.or_brick_majorants(g.ctypes.data, dims, out.ctypes.data, mo)
+        self.lib.or_brick_majorants(g.ctypes.data, dims, b, out.ctypes.data, mo)
         return out, tuple(mo)
 
     def f32_to_f16(self, a):

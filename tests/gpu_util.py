@@ -9,7 +9,10 @@ def first_mismatch(a, b):
         return f"shape {a.shape} != {b.shape}"
     av = a.view(np.uint32) if a.dtype == np.float32 else a
     bv = b.view(np.uint32) if b.dtype == np.float32 else b
-    bad = np.argwhere(av != bv)
+    ne = av != bv
+    if a.dtype == np.float32 and b.dtype == np.float32:
+        ne &= ~(np.isnan(a) & np.isnan(b))  # IEEE-754 leaves the NaN payload and sign open
+    bad = np.argwhere(ne)
     if len(bad) == 0:
         return None
     i = tuple(bad[0])

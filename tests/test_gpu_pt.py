@@ -26,6 +26,7 @@ def small_c1(cfg, S, w=40, h=24, size=48, sv_axis=40, **ext_kw):
 @pytest.mark.parametrize("storage,layout", [(0, 0), (1, 0), (1, 1), (0, 1)])
 def test_fbm_grid_bitexact(ctx, cfg, S, O, storage, layout):
     p = cfg.fbm((37, 29, 23), 1.0 / 37, seed=5, octaves=6)
+    ctx.set_render_ext(cfg.render_ext())
     ctx.set_grid_layout(layout)
     ctx.generate_fbm(p, storage)
     gpu = ctx.download_grid()
@@ -35,6 +36,10 @@ def test_fbm_grid_bitexact(ctx, cfg, S, O, storage, layout):
     assert_bitexact(gpu, ref, f"fbm grid storage={storage} layout={layout}")
     maj, _ = O.brick_majorants(ref)
     assert_bitexact(ctx.majorants(), maj, "brick majorants")
+    for b in (4, 8, 32):
+        ctx.set_render_ext(cfg.render_ext(majorant_brick=b))
+        maj, _ = O.brick_majorants(ref, b)
+        assert_bitexact(ctx.majorants(), maj, f"brick majorants, {b}^3 bricks")
     c = ctx.cloud()
     assert c.maxExtinction == max(float(ref.max()), 0.01)
     assert tuple(c.voxelCount) == (37, 29, 23, 0)
@@ -104,13 +109,16 @@ def test_pt_reference_mode_bitexact(ctx, cfg, S, O):
     assert st.paths == sc.width * sc.height * 3
 
 
-@pytest.mark.parametrize("variant", ["c1", "ratio", "dda", "dda_ratio", "no_nee", "isotropic", "f16_brick"])
+@pytest.mark.parametrize("variant", ["c1", "ratio", "dda", "dda_ratio", "no_nee", "isotropic", "f16_brick", "dda_b8",
+                                     "dda_b32"])
 def test_pt_extensions_bitexact(ctx, cfg, S, O, variant):
     kw = {}
     if variant in ("ratio", "dda_ratio"):
         kw["nee"] = S.CR_NEE_RATIO_TRACKING
-    if variant in ("dda", "dda_ratio", "f16_brick"):
+    if variant in ("dda", "dda_ratio", "f16_brick", "dda_b8", "dda_b32"):
         kw["tracking"] = S.CR_TRACK_BRICK_DDA
+    if variant in ("dda_b8", "dda_b32"):
+        kw["majorant_brick"] = int(variant[5:])
     if variant == "no_nee":
         kw["nee"] = S.CR_NEE_NONE
     sc = small_c1(cfg, S, **kw)

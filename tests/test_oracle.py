@@ -102,6 +102,22 @@ def test_sincosf_accuracy(O):
     assert es < 2.5e-7 and ec < 2.5e-7
 
 
+def test_math_array_domain(O):
+    """sin/cos are defined on |x| <= 2^17 and NaN beyond; the array entry
+    point is the scalar functions element by element, sqrt is IEEE's."""
+    x = np.array([131072.0, -131072.0, 131088.0, np.inf, -np.inf, np.nan, 3.0], np.float32)
+    for op in (3, 4):
+        r = O.math_array(op, x)
+        assert np.isfinite(r[:2]).all() and np.isnan(r[2:6]).all()
+        s, c = C.c_float(), C.c_float()
+        O.lib.or_sincosf(3.0, C.byref(s), C.byref(c))
+        assert r[6] == (s.value if op == 3 else c.value)
+    y = np.random.default_rng(2).random(1000, dtype=np.float32) * 100
+    assert np.array_equal(O.math_array(0, y), np.sqrt(y))
+    assert np.array_equal(O.math_array(1, y), np.array([O.lib.or_logf(float(v)) for v in y], np.float32))
+    assert np.array_equal(O.math_array(2, -y), np.array([O.lib.or_expf(float(-v)) for v in y], np.float32))
+
+
 def test_f16_conversion_matches_ieee(O):
     rng = np.random.default_rng(1)
     vals = np.concatenate([
