@@ -183,6 +183,61 @@ class Context:
     def pt_stats_reset(self):
         self.check(self.lib.cr_pt_stats_reset(self.h), "cr_pt_stats_reset")
 
+    # ------------------------------------------------------- batched render
+    def batch_add(self, frames):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        j = C.c_uint32()
+        self.check(self.lib.cr_batch_add(self.h, arr, len(frames), C.byref(j)), "cr_batch_add")
+        return j.value
+
+    def batch_clear(self):
+        self.check(self.lib.cr_batch_clear(self.h), "cr_batch_clear")
+
+    def batch_size(self):
+        n = C.c_uint32()
+        self.check(self.lib.cr_batch_size(self.h, C.byref(n)), "cr_batch_size")
+        return n.value
+
+    def batch_set_frames(self, job, frames):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        self.check(self.lib.cr_batch_set_frames(self.h, job, arr, len(frames)), "cr_batch_set_frames")
+
+    def batch_clear_images(self):
+        self.check(self.lib.cr_batch_clear_images(self.h), "cr_batch_clear_images")
+
+    def batch_set_tuning(self, block_threads=256, blocks_per_cu=0, tile_width=8):
+        t = structs.BatchTuning(block_threads, blocks_per_cu, tile_width, 0)
+        self.check(self.lib.cr_batch_set_tuning(self.h, C.byref(t)), "cr_batch_set_tuning")
+
+    def batch_tuning(self):
+        t = structs.BatchTuning()
+        self.check(self.lib.cr_batch_get_tuning(self.h, C.byref(t)), "cr_batch_get_tuning")
+        return t
+
+    def batch_render(self):
+        self.check(self.lib.cr_batch_render(self.h), "cr_batch_render")
+
+    def batch_download(self, job):
+        w, h = C.c_uint32(), C.c_uint32()
+        self.check(self.lib.cr_batch_download(self.h, job, None, C.byref(w), C.byref(h)), "cr_batch_download")
+        out = np.empty((h.value, w.value, 4), np.float32)
+        self.check(self.lib.cr_batch_download(self.h, job, _ptr(out), None, None), "cr_batch_download")
+        return out
+
+    def batch_stats(self):
+        s = structs.PtStats()
+        self.check(self.lib.cr_batch_stats_get(self.h, C.byref(s)), "cr_batch_stats_get")
+        return s
+
+    def batch_stats_reset(self):
+        self.check(self.lib.cr_batch_stats_reset(self.h), "cr_batch_stats_reset")
+
+    def batch_autotune(self, tune_bricks=True):
+        t = structs.BatchTuning()
+        r = structs.BatchTuneReport()
+        self.check(self.lib.cr_batch_autotune(self.h, int(tune_bricks), C.byref(t), C.byref(r)), "cr_batch_autotune")
+        return t, r
+
     # -------------------------------------------------------- photon beams
     def trace_beams(self, params, max_beams):
         nb, nd = C.c_uint32(), C.c_uint32()

@@ -61,6 +61,19 @@ SIGNATURES = {
     "cr_pt_upload": (st, [ctx_p, vp]),
     "cr_pt_stats_get": (st, [ctx_p, P(S.PtStats)]),
     "cr_pt_stats_reset": (st, [ctx_p]),
+    "cr_batch_tuning_default": (None, [P(S.BatchTuning)]),
+    "cr_batch_add": (st, [ctx_p, P(S.PushConstants), C.c_uint32, P(C.c_uint32)]),
+    "cr_batch_clear": (st, [ctx_p]),
+    "cr_batch_size": (st, [ctx_p, P(C.c_uint32)]),
+    "cr_batch_set_frames": (st, [ctx_p, C.c_uint32, P(S.PushConstants), C.c_uint32]),
+    "cr_batch_clear_images": (st, [ctx_p]),
+    "cr_batch_set_tuning": (st, [ctx_p, P(S.BatchTuning)]),
+    "cr_batch_get_tuning": (st, [ctx_p, P(S.BatchTuning)]),
+    "cr_batch_render": (st, [ctx_p]),
+    "cr_batch_download": (st, [ctx_p, C.c_uint32, vp, P(C.c_uint32), P(C.c_uint32)]),
+    "cr_batch_stats_get": (st, [ctx_p, P(S.PtStats)]),
+    "cr_batch_stats_reset": (st, [ctx_p]),
+    "cr_batch_autotune": (st, [ctx_p, C.c_int, P(S.BatchTuning), P(S.BatchTuneReport)]),
     "cr_beam_trace_params_default": (None, [P(S.BeamTraceParams)]),
     "cr_ppb_trace_beams": (st, [ctx_p, P(S.BeamTraceParams), C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
     "cr_beams_upload": (st, [ctx_p, vp, C.c_uint32, vp, C.c_uint32]),

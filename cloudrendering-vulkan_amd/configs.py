@@ -8,6 +8,11 @@
       1920x1080, 256 spp, <= 256 bounces, ratio-tracked shadow rays.
   c3  photon beams traced on the c1 volume, 2^22 segments, radius 0.002 of
       the box; 30-bit Morton + 4x8-bit sort + Karras + refit; 1080p gather.
+  c4  64 independent jobs for one batched launch: FP16 cumulus grids with a
+      log-uniform side in [64, 512], square images with a log-uniform side in
+      [128, 1024], log-uniform 16..512 spp, g ~ U[0.6, 0.9], the sun uniform
+      on the upper hemisphere; brick-majorant tracking, ratio-tracked shadow
+      rays, albedo 0.9, <= 16 bounces.
 
 The camera mirrors the reference defaults (UniformBuffers.h:17-26: position
 (0,0,-800), near plane 50, looking down +z) scaled to a unit-size cloud
@@ -173,6 +178,35 @@ def c2(width=1920, height=1080, spp=256, size=(1024, 512, 1024), storage=S.CR_DT
                  storage, layout, 1.0, 64.0, camera(width, height, position=(0.0, 0.0, -0.15)), parameters(0.85),
                  render_ext(albedo=0.999, bounce_cap=bounce_cap, sky=SKY, nee=nee, tracking=tracking, sv_dims=svd),
                  256, spp)
+
+
+def c4(n_jobs=64, seed=4, grid_side=(64, 512), image_side=(128, 1024), spp=(16, 512), camera_z=-0.2):
+    """The jobs of a batched render (a list of Scene), drawn from a seeded RNG."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+
+    def log_uniform(lo, hi):
+        return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+
+    jobs = []
+    for k in range(n_jobs):
+        side = log_uniform(*grid_side)
+        img = log_uniform(*image_side)
+        img += img & 1  # the reference's images are 2 halfWidth x 2 halfHeight
+        n = log_uniform(*spp)
+        g = float(rng.uniform(0.6, 0.9))
+        cos_t = float(rng.uniform(0.0, 1.0))   # sun above the horizon: light travels downward
+        phi = float(rng.uniform(0.0, 2.0 * math.pi))
+        sin_t = math.sqrt(max(0.0, 1.0 - cos_t * cos_t))
+        light = (sin_t * math.cos(phi), -cos_t, sin_t * math.sin(phi))
+        if cos_t < 1e-3:
+            light = (light[0], -1e-3, light[2])
+        cx, cy = (float(v) for v in rng.uniform(-0.1, 0.1, 2))
+        jobs.append(Scene(f"c4/{k}", fbm((side, side, side), 1.0 / side, seed=100 + k, octaves=6), S.CR_DTYPE_F16,
+                          S.CR_GRID_BRICK4, 1.0, 64.0, camera(img, img, position=(cx, cy, camera_z)), parameters(g),
+                          render_ext(albedo=0.9, bounce_cap=16, sky=SKY, nee=S.CR_NEE_RATIO_TRACKING,
+                                     tracking=S.CR_TRACK_BRICK_DDA), 64, n, light_dir=light))
+    return jobs
 
 
 def beam_params(num_paths, seed=38, radius=0.002, albedo=0.9, light_dir=(1.0, -1.0, 0.0), intensity=5.0,

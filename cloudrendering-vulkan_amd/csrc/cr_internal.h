@@ -50,11 +50,27 @@ struct PTArgs {
     uint32_t totalIdx;      // tiles * 64
     uint32_t* queue;        // pixel work counter (zeroed before launch)
     unsigned long long* counters; // paths, vertices, lookups, svLookups, brickVisits
+    uint32_t tileLog2W;     // a wave's claims walk tiles of (1 << tileLog2W) x (64 >> tileLog2W) pixels
+    uint32_t base;          // batched launch: first work index of this job
+};
+
+// One persistent launch over several jobs: the job table (PTArgs per job,
+// queue and counters unused) is copied to LDS by every workgroup; work index
+// i belongs to the last job whose base is <= i.
+struct BatchArgs {
+    const PTArgs* jobs;
+    uint32_t nJobs;
+    uint32_t total;         // sum of the jobs' totalIdx
+    uint32_t* queue;
+    unsigned long long* counters;
 };
 
 // kern_pt.hip
 hipError_t launch_pt(const PTArgs& a, int dtype, int layout, int grid_blocks, hipStream_t s);
 int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee);
+hipError_t launch_pt_batch(const BatchArgs& b, int dtype, int layout, uint32_t tracking, uint32_t nee, int grid_blocks,
+                           int block_threads, hipStream_t s);
+int pt_batch_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, int block_threads, uint32_t n_jobs);
 
 struct SVArgs {
     const void* grid;

@@ -141,6 +141,7 @@ cr_status cr_destroy(cr_context* ctx)
     ppb_free(ctx);
     mlp_free(ctx);
     learn_free(ctx);
+    batch_free(ctx);
     for (auto& ev : ctx->ptEvents) {
         hipEventDestroy(ev.first);
         hipEventDestroy(ev.second);
@@ -620,30 +621,41 @@ cr_status upload_frames(cr_context* ctx, const cr_push_constants* frames, uint32
     return CR_OK;
 }
 
-cr_status fill_pt_args(cr_context* ctx, PTArgs& a, uint32_t n_frames)
+void fill_job_args(PTArgs& a, const DeviceGrid& g, const float* sv, const uint32_t svDims[3],
+                   const cr_camera_properties& cam, const cr_cloud_properties& cloud, const cr_parameters& params,
+                   const cr_shadow_volume_properties& svp, const cr_render_ext& ext, const int32_t* frames,
+                   uint32_t n_frames, float* image, int W, int H, uint32_t tileLog2W)
 {
-    const DeviceGrid& g = ctx->grid;
+    memset(&a, 0, sizeof(a));
     a.grid = g.data; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nbx = g.nbx; a.nby = g.nby;
     a.majorants = g.majorants; a.mx = g.mx; a.my = g.my; a.mz = g.mz;
     const int32_t dims[3] = {g.nx, g.ny, g.nz};
-    for (int k = 0; k < 3; k++)
-        a.bw[k] = ((ctx->cloud.bounds[1][k] - ctx->cloud.bounds[0][k]) * (float)g.mb) / (float)dims[k];
-    a.sv = ctx->sv;
-    a.svx = (int32_t)ctx->svDimsBuilt[0]; a.svy = (int32_t)ctx->svDimsBuilt[1]; a.svz = (int32_t)ctx->svDimsBuilt[2];
-    a.cam = ctx->cam;
-    a.cloud = ctx->cloud;
-    a.params = ctx->params;
-    a.svp = ctx->svp;
-    a.ext = ctx->ext;
-    a.sigmaBar = (ctx->cloud.maxExtinction * ctx->cloud.densityScaling) / ctx->cloud.baseScaling;
-    a.frames = ctx->dFrames;
+    for (int k = 0; k < 3; k++) a.bw[k] = ((cloud.bounds[1][k] - cloud.bounds[0][k]) * (float)g.mb) / (float)dims[k];
+    a.sv = sv;
+    a.svx = (int32_t)svDims[0]; a.svy = (int32_t)svDims[1]; a.svz = (int32_t)svDims[2];
+    a.cam = cam;
+    a.cloud = cloud;
+    a.params = params;
+    a.svp = svp;
+    a.ext = ext;
+    a.sigmaBar = (cloud.maxExtinction * cloud.densityScaling) / cloud.baseScaling;
+    a.frames = frames;
     a.nFrames = n_frames;
-    a.image = (float4*)ctx->image;
-    a.W = ctx->W; a.H = ctx->H;
-    a.padded = 32u * ((uint32_t)ctx->W / 32u + 1u);
-    a.tilesX = (uint32_t)(ctx->W + 7) / 8u;
-    uint32_t tilesY = (uint32_t)(ctx->H + 7) / 8u;
+    a.image = (float4*)image;
+    a.W = W; a.H = H;
+    a.padded = 32u * ((uint32_t)W / 32u + 1u);
+    a.tileLog2W = tileLog2W;
+    const uint32_t tw = 1u << tileLog2W, th = 64u >> tileLog2W;
+    a.tilesX = ((uint32_t)W + tw - 1u) / tw;
+    const uint32_t tilesY = ((uint32_t)H + th - 1u) / th;
     a.totalIdx = a.tilesX * tilesY * 64u;
+    a.base = 0;
+}
+
+cr_status fill_pt_args(cr_context* ctx, PTArgs& a, uint32_t n_frames)
+{
+    fill_job_args(a, ctx->grid, ctx->sv, ctx->svDimsBuilt, ctx->cam, ctx->cloud, ctx->params, ctx->svp, ctx->ext,
+                  ctx->dFrames, n_frames, ctx->image, ctx->W, ctx->H, ctx->ptTileLog2W);
     a.queue = ctx->dQueue;
     a.counters = ctx->dCounters;
     return CR_OK;

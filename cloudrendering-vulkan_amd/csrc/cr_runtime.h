@@ -62,12 +62,15 @@ struct cr_context {
     double ptTotalMs = 0.0;
     uint64_t ptLaunches = 0;
     int ptBlocksOverride = 0;
+    uint32_t ptTileLog2W = 3;                  // 8x8 pixel tiles
 
     // photon beams / LBVH (cr_runtime_ppb.cpp)
     struct Ppb* ppb = nullptr;
     // learned in-scattering network (cr_runtime_mlp.cpp) and its support data (cr_runtime_learn.cpp)
     struct Mlp* mlp = nullptr;
     struct Learn* learn = nullptr;
+    // batched render (cr_runtime_batch.cpp)
+    struct Batch* batch = nullptr;
 };
 
 namespace cr {
@@ -77,10 +80,16 @@ cr_status ensure_buffer(cr_context* ctx, void** p, size_t* cap, size_t bytes, co
 hipError_t zero_async(void* p, size_t bytes, hipStream_t s);
 cr_status upload_frames(cr_context* ctx, const cr_push_constants* frames, uint32_t n);
 int majorant_brick(const cr_render_ext& e);
+void fill_job_args(PTArgs& a, const DeviceGrid& g, const float* sv, const uint32_t svDims[3],
+                   const cr_camera_properties& cam, const cr_cloud_properties& cloud, const cr_parameters& params,
+                   const cr_shadow_volume_properties& svp, const cr_render_ext& ext, const int32_t* frames,
+                   uint32_t n_frames, float* image, int W, int H, uint32_t tileLog2W);
+cr_status fold_pt_events(cr_context* ctx);
 cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb);
 void ppb_free(cr_context* ctx);
 void mlp_free(cr_context* ctx);
 void learn_free(cr_context* ctx);
+void batch_free(cr_context* ctx);
 cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr);
 cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y);
 } // namespace cr

@@ -37,6 +37,7 @@ enum : uint32_t { ST_IDLE = 0, ST_ACTIVE = 1, ST_DEAD = 2 };
 enum : uint32_t { MODE_PRIMARY = 0, MODE_SHADOW = 1 };
 
 struct Lane {
+    uint32_t job;  // batched launch: index into the job table
     int32_t px, py;
     uint32_t id;
     uint32_t f;
@@ -241,20 +242,51 @@ __device__ __forceinline__ void blend(const cr::PTArgs& a, Lane& L)
     L.acc.w = fdiv(L.res.w + L.acc.w * n, d);
 }
 
-template <typename T, int LAYOUT, int TRACK, int NEE>
-__global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a)
+// The job table of a batched launch lives in dynamic LDS.
+template <bool BATCH> __device__ __forceinline__ cr::PTArgs* job_table();
+template <> __device__ __forceinline__ cr::PTArgs* job_table<false>() { return nullptr; }
+template <> __device__ __forceinline__ cr::PTArgs* job_table<true>()
 {
+    extern __shared__ uint64_t pt_job_lds[];
+    return reinterpret_cast<cr::PTArgs*>(pt_job_lds);
+}
+
+// Last job whose first work index is <= idx (jobs are stored by increasing base).
+__device__ __forceinline__ uint32_t find_job(const cr::PTArgs* jobs, uint32_t n, uint32_t idx)
+{
+    uint32_t lo = 0, hi = n - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (jobs[mid].base <= idx) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
+
+// BATCH = false: one job, `a0`.  BATCH = true: the jobs of `b`, one shared
+// work queue over the concatenated pixel ranges; every lane follows its own
+// job's arguments (read from LDS), so lanes of one wave may render different
+// jobs.  Per pixel the arithmetic is the same in both forms.
+template <typename T, int LAYOUT, int TRACK, int NEE, bool BATCH>
+__global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
+{
+    cr::PTArgs* const jobs = job_table<BATCH>();
+    if (BATCH) {
+        const uint32_t words = b.nJobs * (uint32_t)(sizeof(cr::PTArgs) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(b.jobs);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(jobs);
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    uint32_t* const queue = BATCH ? b.queue : a0.queue;
+    const uint32_t totalIdx = BATCH ? b.total : a0.totalIdx;
     Lane L;
     Counters c = {0, 0, 0, 0, 0};
     uint32_t state = ST_IDLE;
-    const f3 ldir = ld3(a.svp.lightDirection);
-    const f3 sdir = neg(ldir);
-    const float I = a.params.lightIntensity;
-    const float albedo = a.ext.albedo;
-    const uint32_t cap = a.ext.bounceCap;
     const uint32_t lane = lane_id();
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     L.mode = MODE_PRIMARY;
+    L.job = 0;
 
     for (;;) {
         // ---- claim pixels for idle lanes (wave-wide, one atomic per wave)
@@ -262,21 +294,26 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a)
         if (idle) {
             uint32_t base = 0;
             const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
-            if (lane == leader) base = atomicAdd(a.queue, (uint32_t)__popcll(idle));
+            if (lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(idle));
             base = __shfl(base, (int)leader, 64);
             if (state == ST_IDLE) {
                 const uint32_t idx = base + (uint32_t)__popcll(idle & lt_mask);
-                if (idx >= a.totalIdx) {
+                if (idx >= totalIdx) {
                     state = ST_DEAD;
                 } else {
-                    const uint32_t tile = idx >> 6, within = idx & 63u;
-                    const int32_t x = (int32_t)((tile % a.tilesX) * 8u + (within & 7u));
-                    const int32_t y = (int32_t)((tile / a.tilesX) * 8u + (within >> 3));
-                    if (x < a.W && y < a.H) {
+                    const uint32_t j = BATCH ? find_job(jobs, b.nJobs, idx) : 0u;
+                    const cr::PTArgs& J = BATCH ? jobs[j] : a0;
+                    const uint32_t local = BATCH ? idx - J.base : idx;
+                    const uint32_t tile = local >> 6, within = local & 63u;
+                    const uint32_t tl = J.tileLog2W, tw = 1u << tl;
+                    const int32_t x = (int32_t)((tile % J.tilesX) * tw + (within & (tw - 1u)));
+                    const int32_t y = (int32_t)((tile / J.tilesX) * (64u >> tl) + (within >> tl));
+                    if (x < J.W && y < J.H) {
+                        L.job = j;
                         L.px = x;
                         L.py = y;
-                        L.id = (uint32_t)x + (uint32_t)y * a.padded;
-                        float4 v = a.image[(size_t)y * a.W + x];
+                        L.id = (uint32_t)x + (uint32_t)y * J.padded;
+                        float4 v = J.image[(size_t)y * J.W + x];
                         L.acc.x = v.x; L.acc.y = v.y; L.acc.z = v.z; L.acc.w = v.w;
                         L.f = 0;
                         state = ST_ACTIVE;
@@ -288,6 +325,13 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a)
         }
         if (__ballot(state != ST_DEAD) == 0ull) break;
         if (state != ST_ACTIVE) continue;
+
+        const cr::PTArgs& a = BATCH ? jobs[L.job] : a0;
+        const f3 ldir = ld3(a.svp.lightDirection);
+        const f3 sdir = neg(ldir);
+        const float I = a.params.lightIntensity;
+        const float albedo = a.ext.albedo;
+        const uint32_t cap = a.ext.bounceCap;
 
         uint32_t ev = EV_NONE;
         if (L.mode != 0xFFu) ev = track_step<T, LAYOUT, TRACK>(a, L, sdir, c);
@@ -387,54 +431,53 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a)
     unsigned long long s0 = wave_sum_u64(c.paths), s1 = wave_sum_u64(c.vertices), s2 = wave_sum_u64(c.lookups);
     unsigned long long s3 = wave_sum_u64(c.svLookups), s4 = wave_sum_u64(c.brickVisits);
     if (lane == 0) {
-        atomicAdd(a.counters + 0, s0);
-        atomicAdd(a.counters + 1, s1);
-        atomicAdd(a.counters + 2, s2);
-        atomicAdd(a.counters + 3, s3);
-        atomicAdd(a.counters + 4, s4);
+        unsigned long long* const cnt = BATCH ? b.counters : a0.counters;
+        atomicAdd(cnt + 0, s0);
+        atomicAdd(cnt + 1, s1);
+        atomicAdd(cnt + 2, s2);
+        atomicAdd(cnt + 3, s3);
+        atomicAdd(cnt + 4, s4);
     }
 }
 
-template <typename T, int LAYOUT, int TRACK, int NEE>
-hipError_t launch_t(const cr::PTArgs& a, int blocks, hipStream_t s)
+using PtFn = void (*)(cr::PTArgs, cr::BatchArgs);
+
+template <typename T, int LAYOUT, bool BATCH>
+PtFn pick_tn(uint32_t tr, uint32_t ne)
 {
-    hipLaunchKernelGGL((pt_kernel<T, LAYOUT, TRACK, NEE>), dim3(blocks), dim3(256), 0, s, a);
-    return hipGetLastError();
+    if (tr == CR_TRACK_BRICK_DDA) {
+        if (ne == CR_NEE_RATIO_TRACKING) return pt_kernel<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_RATIO_TRACKING, BATCH>;
+        if (ne == CR_NEE_NONE) return pt_kernel<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_NONE, BATCH>;
+        return pt_kernel<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_SHADOW_VOLUME, BATCH>;
+    }
+    if (ne == CR_NEE_RATIO_TRACKING) return pt_kernel<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_RATIO_TRACKING, BATCH>;
+    if (ne == CR_NEE_NONE) return pt_kernel<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_NONE, BATCH>;
+    return pt_kernel<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_SHADOW_VOLUME, BATCH>;
 }
 
-template <typename T, int LAYOUT, int TRACK, int NEE>
-int occupancy_t()
+// The kernel instantiation for the grid storage and the render modes.
+template <bool BATCH>
+PtFn pick(int dtype, int layout, uint32_t tr, uint32_t ne)
+{
+    if (dtype == CR_DTYPE_F16) {
+        if (layout == LAYOUT_BRICK4) return pick_tn<__half, LAYOUT_BRICK4, BATCH>(tr, ne);
+        return pick_tn<__half, LAYOUT_DENSE, BATCH>(tr, ne);
+    }
+    if (layout == LAYOUT_BRICK4) return pick_tn<float, LAYOUT_BRICK4, BATCH>(tr, ne);
+    return pick_tn<float, LAYOUT_DENSE, BATCH>(tr, ne);
+}
+
+hipError_t launch(PtFn fn, cr::PTArgs a0, cr::BatchArgs b, int blocks, int threads, size_t lds, hipStream_t s)
+{
+    void* args[] = {&a0, &b};
+    return hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3((unsigned)blocks), dim3((unsigned)threads), args, lds, s);
+}
+
+int occupancy(PtFn fn, int threads, size_t lds)
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pt_kernel<T, LAYOUT, TRACK, NEE>, 256, 0) != hipSuccess) return 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, lds) != hipSuccess) return 1;
     return n > 0 ? n : 1;
-}
-
-template <typename T, int LAYOUT>
-hipError_t dispatch_tn(const cr::PTArgs& a, int blocks, hipStream_t s)
-{
-    const uint32_t tr = a.ext.trackingMode, ne = a.ext.neeMode;
-    if (tr == CR_TRACK_BRICK_DDA) {
-        if (ne == CR_NEE_RATIO_TRACKING) return launch_t<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_RATIO_TRACKING>(a, blocks, s);
-        if (ne == CR_NEE_NONE) return launch_t<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_NONE>(a, blocks, s);
-        return launch_t<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_SHADOW_VOLUME>(a, blocks, s);
-    }
-    if (ne == CR_NEE_RATIO_TRACKING) return launch_t<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_RATIO_TRACKING>(a, blocks, s);
-    if (ne == CR_NEE_NONE) return launch_t<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_NONE>(a, blocks, s);
-    return launch_t<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_SHADOW_VOLUME>(a, blocks, s);
-}
-
-template <typename T, int LAYOUT>
-int occ_tn(uint32_t tr, uint32_t ne)
-{
-    if (tr == CR_TRACK_BRICK_DDA) {
-        if (ne == CR_NEE_RATIO_TRACKING) return occupancy_t<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_RATIO_TRACKING>();
-        if (ne == CR_NEE_NONE) return occupancy_t<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_NONE>();
-        return occupancy_t<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_SHADOW_VOLUME>();
-    }
-    if (ne == CR_NEE_RATIO_TRACKING) return occupancy_t<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_RATIO_TRACKING>();
-    if (ne == CR_NEE_NONE) return occupancy_t<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_NONE>();
-    return occupancy_t<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_SHADOW_VOLUME>();
 }
 
 } // namespace
@@ -443,22 +486,26 @@ namespace cr {
 
 hipError_t launch_pt(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s)
 {
-    if (dtype == CR_DTYPE_F16) {
-        if (layout == LAYOUT_BRICK4) return dispatch_tn<__half, LAYOUT_BRICK4>(a, blocks, s);
-        return dispatch_tn<__half, LAYOUT_DENSE>(a, blocks, s);
-    }
-    if (layout == LAYOUT_BRICK4) return dispatch_tn<float, LAYOUT_BRICK4>(a, blocks, s);
-    return dispatch_tn<float, LAYOUT_DENSE>(a, blocks, s);
+    BatchArgs none = {};
+    return launch(pick<false>(dtype, layout, a.ext.trackingMode, a.ext.neeMode), a, none, blocks, 256, 0, s);
 }
 
 int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee)
 {
-    if (dtype == CR_DTYPE_F16) {
-        if (layout == LAYOUT_BRICK4) return occ_tn<__half, LAYOUT_BRICK4>(tracking, nee);
-        return occ_tn<__half, LAYOUT_DENSE>(tracking, nee);
-    }
-    if (layout == LAYOUT_BRICK4) return occ_tn<float, LAYOUT_BRICK4>(tracking, nee);
-    return occ_tn<float, LAYOUT_DENSE>(tracking, nee);
+    return occupancy(pick<false>(dtype, layout, tracking, nee), 256, 0);
+}
+
+hipError_t launch_pt_batch(const BatchArgs& b, int dtype, int layout, uint32_t tracking, uint32_t nee, int blocks,
+                           int block_threads, hipStream_t s)
+{
+    PTArgs none = {};
+    return launch(pick<true>(dtype, layout, tracking, nee), none, b, blocks, block_threads,
+                  (size_t)b.nJobs * sizeof(PTArgs), s);
+}
+
+int pt_batch_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, int block_threads, uint32_t n_jobs)
+{
+    return occupancy(pick<true>(dtype, layout, tracking, nee), block_threads, (size_t)n_jobs * sizeof(PTArgs));
 }
 
 } // namespace cr

@@ -87,6 +87,17 @@ class GatherStats(C.Structure):
     _fields_ = [("rays", u64), ("nodesVisited", u64), ("beamsTested", u64), ("lastKernelMs", f64)]
 
 
+CR_BATCH_MAX_JOBS = 64
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("blockThreads", u32), ("blocksPerCU", u32), ("tileWidth", u32), ("_reserved", u32)]
+
+
+class BatchTuneReport(C.Structure):
+    _fields_ = [("defaultMs", f32), ("bestMs", f32), ("candidates", u32), ("jobBrick", u32 * CR_BATCH_MAX_JOBS)]
+
+
 SIZES = {
     PushConstants: 24, CameraProperties: 68, CloudProperties: 60, Parameters: 28, ShadowVolumeProperties: 152,
     PhotonBeam: 48, TreeNode: 64, PhotonBeamData: 80, SortElement: 8, LBVHPushConstants: 12,

@@ -112,7 +112,8 @@ enum { CR_LAYOUT_X_FASTEST = 0 /* Grid3D memory order */, CR_LAYOUT_Z_FASTEST = 
  * (FP16 storage is an extension; values convert round-to-nearest-even).
  * Applies SetCloudProperties (main.cpp:144-163) with the context's current
  * baseScaling: bounds from voxel sizes, voxelCount, maxExtinction =
- * max(grid maximum, 0.01).  Also builds the 16^3 brick-majorant grid. */
+ * max(grid maximum, 0.01).  Also builds the brick-majorant grid (bricks of
+ * cr_render_ext.majorantBrick voxels per side, 16 by default). */
 CR_API cr_status cr_grid_upload(cr_context* ctx, const cr_grid_header* hdr, const void* data,
                                 int src_dtype, int layout, int storage_dtype);
 /* Device layout used by the next upload or generation (extension):
@@ -127,8 +128,8 @@ CR_API cr_status cr_grid_load_xyz(cr_context* ctx, const char* path, int storage
 CR_API cr_status cr_grid_save_xyz(cr_context* ctx, const char* path);
 /* Copy the device grid back as FP32, x fastest (sizeX*sizeY*sizeZ floats). */
 CR_API cr_status cr_grid_download(cr_context* ctx, float* out);
-/* Brick-majorant grid (extension): ceil(N/16) per axis, x fastest, max density
- * over each 16^3 brick plus a one-voxel apron. */
+/* Brick-majorant grid (extension): ceil(N/b) per axis for b-voxel bricks, x
+ * fastest, max density over each b^3 brick plus a one-voxel apron. */
 CR_API cr_status cr_grid_majorants(cr_context* ctx, uint32_t dims_out[3], float* out /* may be NULL */);
 
 /* Seeded value-noise fBm cloud generated on the device (extension; the
@@ -209,6 +210,56 @@ typedef struct cr_pt_stats {
 } cr_pt_stats;
 CR_API cr_status cr_pt_stats_get(cr_context* ctx, cr_pt_stats* out);
 CR_API cr_status cr_pt_stats_reset(cr_context* ctx);
+
+/* ---------------------------------------------------- batched render (extension)
+ * Many independent path-tracing jobs, each a complete RenderTechniquePT set-up
+ * (grid, uniforms, shadow volume, image, frame list), rendered by ONE
+ * persistent launch per call: one work queue runs over all jobs' pixels, so
+ * small jobs do not leave the GPU idle.  The launch is recorded once into a
+ * HIP graph and replayed on later calls; frame lists may change between
+ * calls.  Every pixel of a job equals cr_pt_render of the same set-up bit for
+ * bit.  No reference counterpart: the reference renders one technique per
+ * frame (main.cpp:603-735, RenderLoop/DrawFrame). */
+#define CR_BATCH_MAX_JOBS 64
+typedef struct cr_batch_tuning {
+    uint32_t blockThreads; /* workgroup size of the persistent grid: 64, 128 or 256 */
+    uint32_t blocksPerCU;  /* workgroups per CU; 0 = as many as fit */
+    uint32_t tileWidth;    /* width of the 64-pixel tiles a wave's claims walk: 8, 16 or 32 */
+    uint32_t _reserved;
+} cr_batch_tuning;
+typedef struct cr_batch_tune_report {
+    float defaultMs;       /* one frame of every job with the default tuning */
+    float bestMs;          /* the same with the chosen tuning */
+    uint32_t candidates;   /* launch shapes timed */
+    uint32_t jobBrick[CR_BATCH_MAX_JOBS]; /* chosen majorant brick per job (insertion order) */
+} cr_batch_tune_report;
+CR_API void cr_batch_tuning_default(cr_batch_tuning* t);
+/* Moves the context's grid (with its majorants) and shadow volume into a new
+ * job, with copies of the current camera, cloud, parameters, shadow-volume and
+ * extension uniforms and of the n_frames frames; the job gets its own cleared
+ * RGBA32F image.  The context is left without grid and shadow volume.  All
+ * jobs share the grid storage type and layout and the tracking and NEE modes;
+ * at most CR_BATCH_MAX_JOBS jobs (CR_ERR_CAPACITY beyond). */
+CR_API cr_status cr_batch_add(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames, uint32_t* job);
+CR_API cr_status cr_batch_clear(cr_context* ctx); /* frees every job */
+CR_API cr_status cr_batch_size(cr_context* ctx, uint32_t* n_jobs);
+CR_API cr_status cr_batch_set_frames(cr_context* ctx, uint32_t job, const cr_push_constants* frames, uint32_t n_frames);
+CR_API cr_status cr_batch_clear_images(cr_context* ctx);
+CR_API cr_status cr_batch_set_tuning(cr_context* ctx, const cr_batch_tuning* t);
+CR_API cr_status cr_batch_get_tuning(cr_context* ctx, cr_batch_tuning* t);
+/* Renders every job's frames (graph launch; asynchronous like cr_pt_render). */
+CR_API cr_status cr_batch_render(cr_context* ctx);
+/* Copies job's image (W x H RGBA32F) to rgba; width/height may be NULL. */
+CR_API cr_status cr_batch_download(cr_context* ctx, uint32_t job, float* rgba, uint32_t* width, uint32_t* height);
+CR_API cr_status cr_batch_stats_get(cr_context* ctx, cr_pt_stats* out);
+CR_API cr_status cr_batch_stats_reset(cr_context* ctx);
+/* Autotuner: times one frame of every job (into scratch images; the jobs'
+ * images are untouched) for each launch shape (blockThreads x blocksPerCU x
+ * tileWidth) and keeps the fastest.  With tune_bricks, first times each job
+ * alone with majorant bricks of 8, 16 and 32 voxels and keeps the fastest
+ * (this changes the job's extension block: the job then equals cr_pt_render
+ * with that majorantBrick).  Installs and returns the chosen tuning. */
+CR_API cr_status cr_batch_autotune(cr_context* ctx, int tune_bricks, cr_batch_tuning* out, cr_batch_tune_report* report);
 
 
 /* ------------------------------------------------------------ photon beams

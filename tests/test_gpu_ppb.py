@@ -130,7 +130,12 @@ def test_gather_bitexact(ctx, cfg, S, O, ref_compat, jitter):
     b = O.beam_scene(s, sorted_beams, gd, nodes, nb, ref_compat, jitter, 0.9)
     ref = O.gather_render(b, fr)
     assert_bitexact(gpu, ref, f"gather ref_compat={ref_compat} jitter={jitter}")
-    assert gpu[..., 0].max() > 0.0
+    # refCompat keeps PPB_PE.comp's missing guards (no t >= 0 test, a
+    # degenerate beam direction is not rejected), so a pixel may be NaN
+    # there exactly as in the reference; the corrected estimator is finite.
+    if not ref_compat:
+        assert np.isfinite(gpu).all()
+    assert np.nanmax(gpu[..., 0]) > 0.0
     assert st.rays >= sc.width * sc.height * 2
 
 
