@@ -169,10 +169,11 @@ def cpu_baseline(ctx, sc, svp, cfg, seconds):
         grid = grid.astype(np.float16)
     s = O.scene(grid, ctx.cloud(), sc.cam, sc.params, svp, sc.ext, sv=ctx.sv_download(), majorants=ctx.majorants())
     W, H = sc.width, sc.height
-    # calibrate on a band of full rows, then time a block (and frame count) sized to the target
-    w, h = W, max(1, min(H, 16384 // W))
+    # calibrate on one frame of the image (or a centred band of it), then time
+    # a block and frame count sized to the target
+    w, h = W, max(1, min(H, (1 << 21) // W))
     t = time.perf_counter()
-    _, cnt = O.pt_render_region(s, cfg.frames(1), 0, H // 2 - h // 2, w, h, threads)
+    _, cnt = O.pt_render_region(s, cfg.frames(1), 0, (H - h) // 2, w, h, threads)
     dt = time.perf_counter() - t
     per_px = dt / (w * h)
     npx = int(max(256, seconds / max(per_px, 1e-9)))
