@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--spp-per-step", type=int, default=0, help="frames per step; 0 = the workload's full spp")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--via-batch", action="store_true",
+                    help="render through the batched-launch API (one job, HIP-graph replay) instead of cr_pt_render")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pt_c2_traffic.json"))
     return ap.parse_args()
 
@@ -72,19 +74,32 @@ def main():
     frames = cfg.frames(total_frames)
     W, H = sc.width, sc.height
 
+    if args.via_batch:
+        # the grid moves into the job; the baseline below needs it back
+        ctx.batch_clear()
+        ctx.batch_add(frames[:spp])
+
+        def render(i):
+            ctx.batch_set_frames(0, frames[i:i + spp])
+            ctx.batch_render()
+        stats, reset = ctx.batch_stats, ctx.batch_stats_reset
+    else:
+        def render(i):
+            ctx.pt_render(frames[i:i + spp])
+        stats, reset = ctx.pt_stats, ctx.pt_stats_reset
+
     for w in range(args.warmup):
-        ctx.pt_render(frames[w * spp:(w + 1) * spp])
+        render(w * spp)
     ctx.synchronize()
     log(f"warmup {args.warmup} step(s) done")
-    ctx.pt_stats_reset()
+    reset()
     ctx.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
-        i = (args.warmup + k) * spp
-        ctx.pt_render(frames[i:i + spp])
+        render((args.warmup + k) * spp)
     ctx.synchronize()
     t_total = time.perf_counter() - t_start
-    st = ctx.pt_stats()
+    st = stats()
     log(f"timed {args.steps} step(s): {t_total:.3f}s wall, kernel {st.totalKernelMs:.1f} ms over {st.launches} launches")
 
     vertices = st.vertices
@@ -103,6 +118,9 @@ def main():
             traffic = None
 
     cpu = None
+    if args.via_batch:
+        ctx.batch_clear()
+        cloud, svp = sc.setup(ctx)
     if not args.no_cpu:
         cpu = cpu_baseline(ctx, sc, svp, cfg, args.cpu_seconds)
 
@@ -127,6 +145,7 @@ def main():
                          f"{'ratio-tracked shadow rays' if sc.ext.neeMode == S.CR_NEE_RATIO_TRACKING else 'shadow-volume NEE'}, "
                          f"{'brick-majorant DDA' if sc.ext.trackingMode == S.CR_TRACK_BRICK_DDA else 'global majorant'}"),
             "spp_per_step": spp,
+            "launch": "batched API, 1 job, HIP graph" if args.via_batch else "cr_pt_render",
             "image": [W, H],
             "grid": list(sc.fbm.size),
         },

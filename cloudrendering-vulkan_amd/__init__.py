@@ -183,6 +183,56 @@ class Context:
     def pt_stats_reset(self):
         self.check(self.lib.cr_pt_stats_reset(self.h), "cr_pt_stats_reset")
 
+    # ----------------------------------------------------- learned renderer
+    def pt_record(self, frames, H, W):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        out = np.empty((len(frames), H, W, 8), np.float32)
+        self.check(self.lib.cr_pt_record(self.h, arr, len(frames), _ptr(out)), "cr_pt_record")
+        return out
+
+    def pt_continue(self, records, spp):
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, 8)
+        out = np.empty((len(r), 4), np.float32)
+        self.check(self.lib.cr_pt_continue(self.h, _ptr(r), len(r), spp, _ptr(out)), "cr_pt_continue")
+        return out
+
+    def learn_build_mips(self):
+        self.check(self.lib.cr_learn_build_mips(self.h), "cr_learn_build_mips")
+
+    def learn_collect(self, frames, spp, max_points=0):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        n = C.c_uint32()
+        self.check(self.lib.cr_learn_collect(self.h, arr, len(frames), spp, max_points, C.byref(n)), "cr_learn_collect")
+        return n.value
+
+    def learn_dataset_size(self):
+        n = C.c_uint64()
+        self.check(self.lib.cr_learn_dataset_size(self.h, C.byref(n)), "cr_learn_dataset_size")
+        return n.value
+
+    def learn_dataset_clear(self):
+        self.check(self.lib.cr_learn_dataset_clear(self.h), "cr_learn_dataset_clear")
+
+    def learn_dataset(self, first=0, n=None):
+        if n is None:
+            n = self.learn_dataset_size() - first
+        X = np.empty((n, 224), np.uint16)
+        T = np.empty((n, 3), np.float32)
+        self.check(self.lib.cr_learn_dataset_download(self.h, first, n, _ptr(X), _ptr(T)), "cr_learn_dataset_download")
+        return X, T
+
+    def learn_train(self, steps, batch, lr, seed=1):
+        loss = C.c_float()
+        self.check(self.lib.cr_learn_train(self.h, steps, batch, lr, seed, C.byref(loss)), "cr_learn_train")
+        return loss.value
+
+    def learn_render(self, frames):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        self.check(self.lib.cr_learn_render(self.h, arr, len(frames)), "cr_learn_render")
+
+    def mlp_init(self, seed=1):
+        self.check(self.lib.cr_mlp_init(self.h, seed), "cr_mlp_init")
+
     # ------------------------------------------------------- batched render
     def batch_add(self, frames):
         arr = (structs.PushConstants * len(frames))(*frames)

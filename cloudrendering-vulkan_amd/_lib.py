@@ -95,6 +95,14 @@ SIGNATURES = {
     "cr_math_probe": (st, [ctx_p, C.c_int, vp, C.c_uint32, vp]),
     "cr_learn_build_mips": (st, [ctx_p]),
     "cr_learn_features": (st, [ctx_p, vp, C.c_uint32, vp]),
+    "cr_pt_record": (st, [ctx_p, P(S.PushConstants), C.c_uint32, vp]),
+    "cr_pt_continue": (st, [ctx_p, vp, C.c_uint32, C.c_uint32, vp]),
+    "cr_learn_collect": (st, [ctx_p, P(S.PushConstants), C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_uint32)]),
+    "cr_learn_dataset_size": (st, [ctx_p, P(C.c_uint64)]),
+    "cr_learn_dataset_clear": (st, [ctx_p]),
+    "cr_learn_dataset_download": (st, [ctx_p, C.c_uint64, C.c_uint32, vp, vp]),
+    "cr_learn_train": (st, [ctx_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, P(C.c_float)]),
+    "cr_learn_render": (st, [ctx_p, P(S.PushConstants), C.c_uint32]),
 }
 
 

@@ -200,7 +200,7 @@ __device__ __forceinline__ uint64_t voxel_index(const GridView& g, int32_t x, in
 }
 
 template <typename T> __device__ __forceinline__ float load_voxel(const T* p, uint64_t i);
-template <> __device__ __forceinline__ float load_voxel<float>(const float* p, uint64_t i) { return __ldg(p + i); }
+template <> __device__ __forceinline__ float load_voxel<float>(const float* p, uint64_t i) { return p[i]; }
 template <> __device__ __forceinline__ float load_voxel<__half>(const __half* p, uint64_t i)
 {
     const unsigned short* u = reinterpret_cast<const unsigned short*>(p);
@@ -229,14 +229,33 @@ __device__ __forceinline__ float trilinear(const GridView& g, f3 n)
     texel_axis(n.y, g.ny, y0, y1, b);
     texel_axis(n.z, g.nz, z0, z1, c);
     const T* p = reinterpret_cast<const T*>(g.data);
-    float v000 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x0, y0, z0));
-    float v100 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x1, y0, z0));
-    float v010 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x0, y1, z0));
-    float v110 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x1, y1, z0));
-    float v001 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x0, y0, z1));
-    float v101 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x1, y0, z1));
-    float v011 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x0, y1, z1));
-    float v111 = load_voxel<T>(p, voxel_index<LAYOUT>(g, x1, y1, z1));
+    // The voxel index is a sum of one term per axis (grids hold < 2^32 voxels):
+    // dense  x + nx y + nx ny z;  bricks ((x>>2)<<6 | x&3) + (y>>2) 64 nbx +
+    // (y&3) 4 + (z>>2) 64 nbx nby + (z&3) 16.
+    uint32_t ax0, ax1, by0, by1, cz0, cz1;
+    if (LAYOUT == LAYOUT_DENSE) {
+        const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+        ax0 = (uint32_t)x0; ax1 = (uint32_t)x1;
+        by0 = (uint32_t)y0 * sy; by1 = (uint32_t)y1 * sy;
+        cz0 = (uint32_t)z0 * sz; cz1 = (uint32_t)z1 * sz;
+    } else {
+        const uint32_t sy = (uint32_t)g.nbx << 6, sz = ((uint32_t)g.nbx * (uint32_t)g.nby) << 6;
+        ax0 = (((uint32_t)x0 >> 2) << 6) | ((uint32_t)x0 & 3u);
+        ax1 = (((uint32_t)x1 >> 2) << 6) | ((uint32_t)x1 & 3u);
+        by0 = ((uint32_t)y0 >> 2) * sy + (((uint32_t)y0 & 3u) << 2);
+        by1 = ((uint32_t)y1 >> 2) * sy + (((uint32_t)y1 & 3u) << 2);
+        cz0 = ((uint32_t)z0 >> 2) * sz + (((uint32_t)z0 & 3u) << 4);
+        cz1 = ((uint32_t)z1 >> 2) * sz + (((uint32_t)z1 & 3u) << 4);
+    }
+    const uint32_t yz00 = by0 + cz0, yz10 = by1 + cz0, yz01 = by0 + cz1, yz11 = by1 + cz1;
+    float v000 = load_voxel<T>(p, ax0 + yz00);
+    float v100 = load_voxel<T>(p, ax1 + yz00);
+    float v010 = load_voxel<T>(p, ax0 + yz10);
+    float v110 = load_voxel<T>(p, ax1 + yz10);
+    float v001 = load_voxel<T>(p, ax0 + yz01);
+    float v101 = load_voxel<T>(p, ax1 + yz01);
+    float v011 = load_voxel<T>(p, ax0 + yz11);
+    float v111 = load_voxel<T>(p, ax1 + yz11);
     float oa = 1.0f - a, ob = 1.0f - b, oc = 1.0f - c;
     float c00 = v000 * oa + v100 * a;
     float c10 = v010 * oa + v110 * a;

@@ -52,6 +52,11 @@ struct PTArgs {
     unsigned long long* counters; // paths, vertices, lookups, svLookups, brickVisits
     uint32_t tileLog2W;     // a wave's claims walk tiles of (1 << tileLog2W) x (64 >> tileLog2W) pixels
     uint32_t base;          // batched launch: first work index of this job
+    // learned-renderer passes
+    float* records;         // record pass: [nFrames][H][W] records of 8 floats
+    const float* points;    // continuation pass: records to continue
+    float4* out;            // continuation pass: mean radiance per record
+    uint32_t spp;           // continuation pass: samples per record
 };
 
 // One persistent launch over several jobs: the job table (PTArgs per job,
@@ -71,6 +76,10 @@ int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee);
 hipError_t launch_pt_batch(const BatchArgs& b, int dtype, int layout, uint32_t tracking, uint32_t nee, int grid_blocks,
                            int block_threads, hipStream_t s);
 int pt_batch_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, int block_threads, uint32_t n_jobs);
+// kern_pt_learn.hip
+hipError_t launch_pt_record(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s);
+hipError_t launch_pt_continue(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s);
+int pt_learn_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, int continuation);
 
 struct SVArgs {
     const void* grid;

@@ -27,13 +27,21 @@ struct ComposeArgs {
     uint32_t pixels;
     uint32_t nFrames;
     const int32_t* frames;  // (seed, frameCount) pairs
-    const float4* sampleRes;// [frames][pixels]: background radiance, w = 1 when the ray collided
-    const float* Y;         // [frames][pixels][4] network outputs
+    const float* records;   // [frames][pixels] records of the record pass (8 floats)
+    const uint32_t* slot;   // [frames][pixels]: row of Y, or ~0 for an escaped path
+    const float* Y;         // [rows][4] network outputs
 };
 
 hipError_t launch_mip(const float* src, int32_t sx, int32_t sy, int32_t sz, float* dst, int32_t dx, int32_t dy, int32_t dz,
                       hipStream_t s);
 hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
 hipError_t launch_compose(const ComposeArgs& a, hipStream_t s);
+hipError_t launch_count(const float* rec, uint64_t n, uint32_t* counts, hipStream_t s);
+hipError_t launch_compact(const float* rec, uint64_t n, const uint32_t* offsets, float* pts, uint32_t* slot, hipStream_t s);
+hipError_t launch_select(const float* src, uint32_t stride, uint32_t count, float* dst, hipStream_t s);
+hipError_t launch_append(const uint16_t* X, const float4* T4, uint32_t stride, uint32_t count, uint16_t* dsX, float* dsT,
+                         uint64_t offset, hipStream_t s);
+hipError_t launch_batch_gather(const uint16_t* dsX, const float* dsT, uint64_t n, uint32_t seed, uint32_t step, uint32_t B,
+                               uint16_t* X, float* T, hipStream_t s);
 
 } // namespace cr

@@ -186,6 +186,7 @@ cr_status grid_alloc(cr_context* ctx, int32_t nx, int32_t ny, int32_t nz, int dt
     int layout = ctx->gridLayout;
     int32_t nbx = (nx + 3) / 4, nby = (ny + 3) / 4, nbz = (nz + 3) / 4;
     size_t elems = layout == CR_GRID_BRICK4 ? (size_t)nbx * nby * nbz * 64 : (size_t)nx * ny * nz;
+    if (elems >= (1ull << 32)) return fail(ctx, CR_ERR_CAPACITY, "grids hold fewer than 2^32 voxels (32-bit lookups)");
     size_t bytes = elems * (dtype == CR_DTYPE_F16 ? 2 : 4);
     if (g.data && g.bytes != bytes) {
         hipFree(g.data);
@@ -263,6 +264,7 @@ cr_status grid_finish(cr_context* ctx, double vx, double vy, double vz)
     c.bounds[1][2] = -c.bounds[0][2] + sz;
     c.bounds[1][3] = -c.bounds[0][3] + 0.0f;
     ctx->hasGrid = true;
+    ctx->gridVersion++;
     ctx->svValid = false;
     return CR_OK;
 }
@@ -542,6 +544,8 @@ cr_status cr_sv_build(cr_context* ctx)
     hipSetDevice(ctx->device);
     uint32_t d[3];
     sv_dims_of(ctx, d);
+    if ((uint64_t)d[0] * d[1] * d[2] >= (1ull << 32))
+        return fail(ctx, CR_ERR_CAPACITY, "shadow volumes hold fewer than 2^32 texels (32-bit lookups)");
     size_t bytes = (size_t)d[0] * d[1] * d[2] * 4;
     cr_status st = ensure_buffer(ctx, (void**)&ctx->sv, &ctx->svCap, bytes, "shadow volume");
     if (st != CR_OK) return st;

@@ -23,6 +23,7 @@ struct cr_context {
     cr::DeviceGrid grid;
     int gridLayout = CR_GRID_DENSE;
     bool hasGrid = false;
+    uint64_t gridVersion = 0;   // bumped by every grid load or generation
     float gridMax = 0.0f;
     double voxelSize[3] = {0, 0, 0};
 
@@ -92,4 +93,6 @@ void learn_free(cr_context* ctx);
 void batch_free(cr_context* ctx);
 cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr);
 cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y);
+cr_status mlp_last_loss(cr_context* ctx, float* loss);
+cr_status fill_pt_args(cr_context* ctx, PTArgs& a, uint32_t n_frames);
 } // namespace cr

@@ -2,6 +2,7 @@
 // mip pyramid and stencil features.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -16,11 +17,31 @@ struct Learn {
     size_t mipCap[LEARN_LEVELS] = {};
     int32_t dims[LEARN_LEVELS][3] = {};
     bool valid = false;
-    const void* builtFrom = nullptr;
+    uint64_t builtVersion = 0;
     float* points = nullptr;
     size_t pointsCap = 0;
     uint16_t* X = nullptr;
     size_t XCap = 0;
+    // record pass, compaction, network outputs
+    float* records = nullptr;
+    size_t recordsCap = 0;
+    uint32_t* slot = nullptr;
+    size_t slotCap = 0;
+    uint32_t* counts = nullptr;
+    size_t countsCap = 0;
+    float* Y = nullptr;
+    size_t YCap = 0;
+    float* targets = nullptr;
+    size_t targetsCap = 0;
+    // training data set and the minibatch buffers
+    uint16_t* dsX = nullptr;
+    float* dsT = nullptr;
+    uint64_t dsN = 0, dsCap = 0;
+    uint16_t* bX = nullptr;
+    size_t bXCap = 0;
+    float* bT = nullptr;
+    size_t bTCap = 0;
+    uint32_t trainSteps = 0;
 };
 
 namespace cr {
@@ -31,8 +52,9 @@ void learn_free(cr_context* ctx)
     if (!L) return;
     for (int k = 0; k < LEARN_LEVELS; k++)
         if (L->mips[k]) hipFree(L->mips[k]);
-    if (L->points) hipFree(L->points);
-    if (L->X) hipFree(L->X);
+    void* bufs[] = {L->points, L->X, L->records, L->slot, L->counts, L->Y, L->targets, L->dsX, L->dsT, L->bX, L->bT};
+    for (void* b : bufs)
+        if (b) hipFree(b);
     delete L;
     ctx->learn = nullptr;
 }
@@ -62,7 +84,7 @@ cr_status learn_mips(cr_context* ctx)
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "mip pyramid");
     L->valid = true;
-    L->builtFrom = g.data;
+    L->builtVersion = ctx->gridVersion;
     return CR_OK;
 }
 
@@ -70,7 +92,8 @@ cr_status learn_mips(cr_context* ctx)
 cr_status learn_features_device(cr_context* ctx, const float* points, uint64_t n, uint16_t* X)
 {
     Learn* L = learn_get(ctx);
-    if (!L->valid) return fail(ctx, CR_ERR_STATE, "mip pyramid not built (cr_learn_build_mips)");
+    if (!L->valid || L->builtVersion != ctx->gridVersion)
+        return fail(ctx, CR_ERR_STATE, "mip pyramid not built for the current grid (cr_learn_build_mips)");
     FeatureArgs a;
     memset(&a, 0, sizeof(a));
     a.points = points;
@@ -117,6 +140,288 @@ cr_status cr_learn_features(cr_context* ctx, const float* points, uint32_t n, ui
     e = hipMemcpyAsync(X, L->X, (size_t)n * 448, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return e == hipSuccess ? CR_OK : hip_fail(ctx, e, "copy features");
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------------
+// Learned renderer: record pass, continuation pass, compaction, training data,
+// training loop and the learned render.
+
+#define CR_HIP(ctx, call, what) \
+    do { hipError_t _e = (call); if (_e != hipSuccess) return hip_fail((ctx), _e, (what)); } while (0)
+#define CR_TRY(call) \
+    do { cr_status _s = (call); if (_s != CR_OK) return _s; } while (0)
+
+namespace {
+
+cr_status check_render_state(cr_context* ctx)
+{
+    if (!ctx->hasGrid) return fail(ctx, CR_ERR_STATE, "no grid");
+    if (!ctx->image) return fail(ctx, CR_ERR_STATE, "no camera set");
+    if (ctx->ext.neeMode == CR_NEE_SHADOW_VOLUME && !ctx->svValid)
+        return fail(ctx, CR_ERR_STATE, "shadow-volume NEE needs cr_sv_build after the last grid/shadow-volume change");
+    return CR_OK;
+}
+
+int pass_blocks(cr_context* ctx, int continuation, uint32_t work)
+{
+    const int per_cu = pt_learn_blocks_per_cu(ctx->grid.dtype, ctx->grid.layout, ctx->ext.trackingMode, ctx->ext.neeMode,
+                                              continuation);
+    int blocks = ctx->numCUs * per_cu;
+    const uint32_t needed = (work + 255u) / 256u;
+    if ((uint32_t)blocks > needed) blocks = (int)needed;
+    return blocks > 0 ? blocks : 1;
+}
+
+// Record pass of n_frames frames into L->records; returns the record count.
+cr_status record_pass(cr_context* ctx, Learn* L, const cr_push_constants* frames, uint32_t n_frames, uint64_t* n_out)
+{
+    CR_TRY(check_render_state(ctx));
+    CR_TRY(upload_frames(ctx, frames, n_frames));
+    const uint64_t n = (uint64_t)n_frames * (uint64_t)ctx->W * (uint64_t)ctx->H;
+    CR_TRY(ensure_buffer(ctx, (void**)&L->records, &L->recordsCap, n * 32, "records"));
+    PTArgs a;
+    CR_TRY(fill_pt_args(ctx, a, n_frames));
+    a.records = L->records;
+    a.counters = ctx->dCounters + 8;
+    CR_HIP(ctx, zero_async(ctx->dQueue, 4, ctx->stream), "zero queue");
+    CR_HIP(ctx, launch_pt_record(a, ctx->grid.dtype, ctx->grid.layout, pass_blocks(ctx, 0, a.totalIdx), ctx->stream),
+           "record pass");
+    *n_out = n;
+    return CR_OK;
+}
+
+// Continuation pass over m device records into out (float4 per record).
+cr_status continue_pass(cr_context* ctx, const float* pts, uint32_t m, uint32_t spp, float* out)
+{
+    if (m == 0) return CR_OK;
+    PTArgs a;
+    CR_TRY(fill_pt_args(ctx, a, 0));
+    a.points = pts;
+    a.out = reinterpret_cast<float4*>(out);
+    a.spp = spp;
+    a.totalIdx = m;
+    a.counters = ctx->dCounters + 8;
+    CR_HIP(ctx, zero_async(ctx->dQueue, 4, ctx->stream), "zero queue");
+    CR_HIP(ctx, launch_pt_continue(a, ctx->grid.dtype, ctx->grid.layout, pass_blocks(ctx, 1, m), ctx->stream),
+           "continuation pass");
+    return CR_OK;
+}
+
+// Collided records of L->records (n) into L->points in record order, with
+// L->slot[i] = row or ~0; returns the count m (synchronises the stream).
+cr_status compact(cr_context* ctx, Learn* L, uint64_t n, uint32_t* m_out)
+{
+    const uint64_t nb = (n + 255) / 256;
+    CR_TRY(ensure_buffer(ctx, (void**)&L->counts, &L->countsCap, nb * 4, "counts"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->slot, &L->slotCap, n * 4, "slots"));
+    CR_HIP(ctx, launch_count(L->records, n, L->counts, ctx->stream), "count collisions");
+    std::vector<uint32_t> c(nb);
+    CR_HIP(ctx, hipMemcpyAsync(c.data(), L->counts, nb * 4, hipMemcpyDeviceToHost, ctx->stream), "copy counts");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    uint64_t total = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        const uint32_t v = c[b];
+        c[b] = (uint32_t)total;
+        total += v;
+    }
+    if (total > 0xFFFFFFFEull) return fail(ctx, CR_ERR_CAPACITY, "more than 2^32-2 collisions in one pass");
+    CR_HIP(ctx, hipMemcpyAsync(L->counts, c.data(), nb * 4, hipMemcpyHostToDevice, ctx->stream), "copy offsets");
+    CR_TRY(ensure_buffer(ctx, (void**)&L->points, &L->pointsCap, (total ? total : 1) * 32, "points"));
+    CR_HIP(ctx, launch_compact(L->records, n, L->counts, L->points, L->slot, ctx->stream), "compact collisions");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");  // c is read by the copy above
+    *m_out = (uint32_t)total;
+    return CR_OK;
+}
+
+cr_status ensure_mips(cr_context* ctx, Learn* L)
+{
+    if (L->valid && L->builtVersion == ctx->gridVersion) return CR_OK;
+    return learn_mips(ctx);
+}
+
+cr_status ensure_dataset(cr_context* ctx, Learn* L, uint64_t need)
+{
+    if (need <= L->dsCap) return CR_OK;
+    uint64_t cap = std::max<uint64_t>(need, L->dsCap * 2);
+    cap = std::max<uint64_t>(cap, 1u << 16);
+    uint16_t* X = nullptr;
+    float* T = nullptr;
+    CR_HIP(ctx, hipMalloc((void**)&X, cap * 448), "hipMalloc data set");
+    hipError_t e = hipMalloc((void**)&T, cap * 12);
+    if (e != hipSuccess) {
+        hipFree(X);
+        return hip_fail(ctx, e, "hipMalloc data set");
+    }
+    if (L->dsN) {
+        e = hipMemcpyAsync(X, L->dsX, L->dsN * 448, hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(T, L->dsT, L->dsN * 12, hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            hipFree(X);
+            hipFree(T);
+            return hip_fail(ctx, e, "grow data set");
+        }
+    }
+    if (L->dsX) hipFree(L->dsX);
+    if (L->dsT) hipFree(L->dsT);
+    L->dsX = X;
+    L->dsT = T;
+    L->dsCap = cap;
+    return CR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+cr_status cr_pt_record(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames, float* records)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    if (!frames || n_frames == 0 || !records) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null frames or output");
+    hipSetDevice(ctx->device);
+    Learn* L = learn_get(ctx);
+    uint64_t n = 0;
+    CR_TRY(record_pass(ctx, L, frames, n_frames, &n));
+    CR_HIP(ctx, hipMemcpyAsync(records, L->records, n * 32, hipMemcpyDeviceToHost, ctx->stream), "copy records");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return CR_OK;
+}
+
+cr_status cr_pt_continue(cr_context* ctx, const float* records, uint32_t n, uint32_t spp, float* out)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    if (!records || !out) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null records or output");
+    if (spp == 0) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "spp must be >= 1");
+    if (n == 0) return CR_OK;
+    CR_TRY(check_render_state(ctx));
+    hipSetDevice(ctx->device);
+    Learn* L = learn_get(ctx);
+    CR_TRY(ensure_buffer(ctx, (void**)&L->points, &L->pointsCap, (size_t)n * 32, "points"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->targets, &L->targetsCap, (size_t)n * 16, "targets"));
+    CR_HIP(ctx, hipMemcpyAsync(L->points, records, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream), "copy records");
+    CR_TRY(continue_pass(ctx, L->points, n, spp, L->targets));
+    CR_HIP(ctx, hipMemcpyAsync(out, L->targets, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream), "copy output");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return CR_OK;
+}
+
+cr_status cr_learn_collect(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames, uint32_t spp,
+                           uint32_t max_points, uint32_t* added)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    if (!frames || n_frames == 0) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null frames");
+    if (spp == 0) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "spp must be >= 1");
+    if (added) *added = 0;
+    hipSetDevice(ctx->device);
+    Learn* L = learn_get(ctx);
+    CR_TRY(check_render_state(ctx));
+    CR_TRY(ensure_mips(ctx, L));
+    uint64_t n = 0;
+    uint32_t m = 0;
+    CR_TRY(record_pass(ctx, L, frames, n_frames, &n));
+    CR_TRY(compact(ctx, L, n, &m));
+    if (m == 0) return CR_OK;
+    const uint32_t stride = (max_points == 0 || m <= max_points) ? 1u : (m + max_points - 1u) / max_points;
+    const uint32_t count = (m + stride - 1u) / stride;
+    // the selected collisions, contiguous (the record buffer is free again)
+    float* sel = L->points;
+    if (stride > 1) {
+        CR_HIP(ctx, launch_select(L->points, stride, count, L->records, ctx->stream), "select");
+        sel = L->records;
+    }
+    CR_TRY(ensure_buffer(ctx, (void**)&L->targets, &L->targetsCap, (size_t)count * 16, "targets"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->X, &L->XCap, (size_t)count * 448, "features"));
+    CR_TRY(continue_pass(ctx, sel, count, spp, L->targets));
+    CR_TRY(learn_features_device(ctx, sel, count, L->X));
+    CR_TRY(ensure_dataset(ctx, L, L->dsN + count));
+    CR_HIP(ctx, launch_append(L->X, reinterpret_cast<const float4*>(L->targets), 1, count, L->dsX, L->dsT, L->dsN,
+                              ctx->stream), "append");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    L->dsN += count;
+    if (added) *added = count;
+    return CR_OK;
+}
+
+cr_status cr_learn_dataset_size(cr_context* ctx, uint64_t* n)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    if (!n) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null output");
+    *n = ctx->learn ? ctx->learn->dsN : 0;
+    return CR_OK;
+}
+
+cr_status cr_learn_dataset_clear(cr_context* ctx)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    if (ctx->learn) ctx->learn->dsN = 0;
+    return CR_OK;
+}
+
+cr_status cr_learn_dataset_download(cr_context* ctx, uint64_t first, uint32_t n, uint16_t* X, float* T)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    Learn* L = learn_get(ctx);
+    if (first + n > L->dsN) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "rows beyond the data set");
+    if (n == 0) return CR_OK;
+    if (X) CR_HIP(ctx, hipMemcpyAsync(X, L->dsX + first * 224, (size_t)n * 448, hipMemcpyDeviceToHost, ctx->stream), "copy X");
+    if (T) CR_HIP(ctx, hipMemcpyAsync(T, L->dsT + first * 3, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "copy T");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return CR_OK;
+}
+
+cr_status cr_learn_train(cr_context* ctx, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* loss)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    if (batch == 0 || batch > (1u << 22)) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "batch must be in [1, 2^22]");
+    if (!(lr > 0.0f)) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "lr must be > 0");
+    hipSetDevice(ctx->device);
+    Learn* L = learn_get(ctx);
+    if (L->dsN == 0) return fail(ctx, CR_ERR_STATE, "empty training data set (cr_learn_collect)");
+    if (!ctx->mlp) CR_TRY(cr_mlp_init(ctx, seed));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->bX, &L->bXCap, (size_t)batch * 448, "batch X"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->bT, &L->bTCap, (size_t)batch * 12, "batch T"));
+    for (uint32_t s = 0; s < steps; s++) {
+        CR_HIP(ctx, launch_batch_gather(L->dsX, L->dsT, L->dsN, seed, L->trainSteps + s, batch, L->bX, L->bT, ctx->stream),
+               "batch gather");
+        CR_TRY(mlp_train_device(ctx, L->bX, L->bT, batch, lr));
+    }
+    L->trainSteps += steps;
+    if (loss) CR_TRY(mlp_last_loss(ctx, loss));
+    return CR_OK;
+}
+
+cr_status cr_learn_render(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames)
+{
+    if (!ctx) return CR_ERR_INVALID_ARGUMENT;
+    if (!frames && n_frames) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null frames");
+    if (n_frames == 0) return CR_OK;
+    if (!ctx->mlp) return fail(ctx, CR_ERR_STATE, "no network (cr_mlp_init / cr_learn_train)");
+    hipSetDevice(ctx->device);
+    Learn* L = learn_get(ctx);
+    CR_TRY(check_render_state(ctx));
+    CR_TRY(ensure_mips(ctx, L));
+    uint64_t n = 0;
+    uint32_t m = 0;
+    CR_TRY(record_pass(ctx, L, frames, n_frames, &n));
+    CR_TRY(compact(ctx, L, n, &m));
+    if (m > 0) {
+        CR_TRY(ensure_buffer(ctx, (void**)&L->X, &L->XCap, (size_t)m * 448, "features"));
+        CR_TRY(ensure_buffer(ctx, (void**)&L->Y, &L->YCap, (size_t)m * 16, "network outputs"));
+        CR_TRY(learn_features_device(ctx, L->points, m, L->X));
+        CR_TRY(mlp_forward_device(ctx, L->X, m, L->Y));
+    }
+    ComposeArgs a;
+    a.image = reinterpret_cast<float4*>(ctx->image);
+    a.pixels = (uint32_t)(ctx->W * ctx->H);
+    a.nFrames = n_frames;
+    a.frames = ctx->dFrames;
+    a.records = L->records;
+    a.slot = L->slot;
+    a.Y = L->Y;
+    CR_HIP(ctx, launch_compose(a, ctx->stream), "compose");
+    return CR_OK;
 }
 
 } // extern "C"

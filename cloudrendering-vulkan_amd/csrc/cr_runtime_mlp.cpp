@@ -327,6 +327,16 @@ cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t* B, fl
 
 namespace cr {
 
+// Loss of the last training step (synchronises the stream).
+cr_status mlp_last_loss(cr_context* ctx, float* loss)
+{
+    Mlp* m;
+    CR_TRY(mlp_get(ctx, &m));
+    CR_HIP(ctx, hipMemcpyAsync(loss, m->loss, 4, hipMemcpyDeviceToHost, ctx->stream), "copy loss");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return CR_OK;
+}
+
 // Forward (training), loss, backward and Adam on device-resident X and T.
 cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr)
 {

@@ -89,8 +89,9 @@ __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
     for (int q = 0; q < 28; q++) o4[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
 }
 
-// Per pixel, frames in order: L = escaped ? background : expm1(network) and
-// the reference's running mean (PathTracer.comp:420-424).
+// Per pixel, frames in order: L = the escaped path's radiance, or
+// expm1(network) at its first collision, blended with the reference's running
+// mean (PathTracer.comp:420-424).
 __global__ void __launch_bounds__(256) compose_kernel(cr::ComposeArgs a)
 {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
@@ -98,17 +99,16 @@ __global__ void __launch_bounds__(256) compose_kernel(cr::ComposeArgs a)
     float4 acc = a.image[p];
     for (uint32_t f = 0; f < a.nFrames; f++) {
         const size_t s = (size_t)f * a.pixels + p;
-        const float4 e = a.sampleRes[s];  // w = 1 when the camera ray collided
+        const uint32_t slot = a.slot[s];
         float4 L;
-        if (e.w > 0.5f) {
-            const float* y = a.Y + s * 4;
+        if (slot != 0xFFFFFFFFu) {
+            const float* y = a.Y + (size_t)slot * 4;
             L.x = fmaxf(expm1f(y[0]), 0.0f);
             L.y = fmaxf(expm1f(y[1]), 0.0f);
             L.z = fmaxf(expm1f(y[2]), 0.0f);
             L.w = 1.0f;
         } else {
-            L = e;
-            L.w = 1.0f;
+            L = reinterpret_cast<const float4*>(a.records)[2 * s];
         }
         const uint32_t fc = (uint32_t)a.frames[2 * f + 1];
         const float n = (float)fc, d = (float)(fc + 1u);
@@ -120,9 +120,130 @@ __global__ void __launch_bounds__(256) compose_kernel(cr::ComposeArgs a)
     a.image[p] = acc;
 }
 
+// Stream compaction of the collided records, in record order (deterministic):
+// per-block counts, a host-side exclusive scan, then the write pass.
+__device__ __forceinline__ bool collided(const float* rec, uint64_t i) { return rec[i * 8 + 7] > 0.5f; }
+
+__global__ void __launch_bounds__(256) count_kernel(const float* rec, uint64_t n, uint32_t* counts)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool f = i < n && collided(rec, i);
+    __shared__ uint32_t wc[4];
+    const unsigned long long bal = __ballot(f);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+__global__ void __launch_bounds__(256) compact_kernel(const float* rec, uint64_t n, const uint32_t* offsets, float* pts,
+                                                      uint32_t* slot)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool f = i < n && collided(rec, i);
+    __shared__ uint32_t wc[4];
+    const unsigned long long bal = __ballot(f);
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) wc[w] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    if (i >= n) return;
+    if (!f) {
+        slot[i] = 0xFFFFFFFFu;
+        return;
+    }
+    uint32_t s = offsets[blockIdx.x];
+    for (uint32_t k = 0; k < w; k++) s += wc[k];
+    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    s += (uint32_t)__popcll(bal & lt);
+    slot[i] = s;
+    const float4* src = reinterpret_cast<const float4*>(rec + i * 8);
+    float4* dst = reinterpret_cast<float4*>(pts + (size_t)s * 8);
+    dst[0] = src[0];
+    dst[1] = src[1];
+}
+
+// Rows k*stride (k < count) of 8-float records copied to dst[k].
+__global__ void __launch_bounds__(256) select_kernel(const float* src, uint32_t stride, uint32_t count, float* dst)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= count) return;
+    const float4* s4 = reinterpret_cast<const float4*>(src + (size_t)k * stride * 8);
+    float4* d4 = reinterpret_cast<float4*>(dst + (size_t)k * 8);
+    d4[0] = s4[0];
+    d4[1] = s4[1];
+}
+
+// Rows k*stride (k < count) of X and T4 appended to the data set at `offset`.
+__global__ void __launch_bounds__(256) append_kernel(const uint16_t* X, const float4* T4, uint32_t stride, uint32_t count,
+                                                     uint16_t* dsX, float* dsT, uint64_t offset)
+{
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= count) return;
+    const uint64_t src = (uint64_t)k * stride, dst = offset + k;
+    const uint4* xs = reinterpret_cast<const uint4*>(X + src * 224);
+    uint4* xd = reinterpret_cast<uint4*>(dsX + dst * 224);
+    for (int q = 0; q < 28; q++) xd[q] = xs[q];
+    const float4 t = T4[src];
+    dsT[dst * 3 + 0] = t.x;
+    dsT[dst * 3 + 1] = t.y;
+    dsT[dst * 3 + 2] = t.z;
+}
+
+// A training batch: row r is data-set row hash(seed, step, r) mod n.
+__global__ void __launch_bounds__(256) batch_gather_kernel(const uint16_t* dsX, const float* dsT, uint64_t n, uint32_t seed,
+                                                           uint32_t step, uint32_t B, uint16_t* X, float* T)
+{
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= B) return;
+    uint32_t state = (seed * 0x9E3779B9u) ^ (step * 0x85EBCA6Bu) ^ (r * 0xC2B2AE35u);
+    state = state * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    word = (word >> 22u) ^ word;
+    const uint64_t src = (uint64_t)word % n;
+    const uint4* xs = reinterpret_cast<const uint4*>(dsX + src * 224);
+    uint4* xd = reinterpret_cast<uint4*>(X + (uint64_t)r * 224);
+    for (int q = 0; q < 28; q++) xd[q] = xs[q];
+    for (int c = 0; c < 3; c++) T[(uint64_t)r * 3 + c] = dsT[src * 3 + c];
+}
+
 } // namespace
 
 namespace cr {
+
+hipError_t launch_count(const float* rec, uint64_t n, uint32_t* counts, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rec, n, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(const float* rec, uint64_t n, const uint32_t* offsets, float* pts, uint32_t* slot, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rec, n, offsets, pts, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(const float* src, uint32_t stride, uint32_t count, float* dst, hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(select_kernel, dim3((count + 255) / 256), dim3(256), 0, s, src, stride, count, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_append(const uint16_t* X, const float4* T4, uint32_t stride, uint32_t count, uint16_t* dsX, float* dsT,
+                         uint64_t offset, hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(append_kernel, dim3((count + 255) / 256), dim3(256), 0, s, X, T4, stride, count, dsX, dsT, offset);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_gather(const uint16_t* dsX, const float* dsT, uint64_t n, uint32_t seed, uint32_t step, uint32_t B,
+                               uint16_t* X, float* T, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_gather_kernel, dim3((B + 255) / 256), dim3(256), 0, s, dsX, dsT, n, seed, step, B, X, T);
+    return hipGetLastError();
+}
 
 hipError_t launch_mip(const float* src, int32_t sx, int32_t sy, int32_t sz, float* dst, int32_t dx, int32_t dy, int32_t dz,
                       hipStream_t s)

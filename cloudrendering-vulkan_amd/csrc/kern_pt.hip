@@ -1,511 +1,30 @@
-// Persistent path-tracing kernel for gfx950.
-//
-// Computes, per pixel and per frame, what shaders/PathTracer.comp main
-// (:366-425) computes, and blends the frames into the RGBA32F accumulation
-// image with the reference's running mean (:420-424).
-//
-// Shape of the computation on CDNA4:
-//  * Persistent grid: a few waves per SIMD loop until a global pixel queue is
-//    drained.  Lanes claim pixels individually: the wave ballots its idle
-//    lanes, one lane advances the queue counter by the popcount, and each
-//    lane takes base + its rank among the idle lanes.  Claimed indices walk
-//    8x8 pixel tiles so neighbouring lanes trace neighbouring pixels.
-//  * Path regeneration: a lane renders all frames of its pixel back to back
-//    (frame f uses initializeRandom(seed_f * id) exactly as the reference),
-//    keeping the running mean in registers: 32 B of image traffic per pixel
-//    per launch.
-//  * Flattened state machine: every loop iteration performs at most one
-//    density lookup per lane (a delta-tracking step of the camera path, or a
-//    ratio-tracking step of a shadow ray); collisions, escapes, frame and
-//    pixel changes are handled in a divergent tail.  Lanes never wait at the
-//    end of an inner tracking loop for the slowest lane of the wave.
-//  * Optional brick-majorant DDA (16^3 bricks with a one-voxel apron) skips
-//    empty space and lowers the majorant in thin regions.
-// The sequence of floating-point operations and RNG draws of every path is
-// the one fixed by the reference shader (in the order the GLSL states it),
-// so for the same inputs each pixel's value is reproducible bit for bit.
-#include "cr_device.h"
-#include "cr_internal.h"
-
-using namespace crd;
-
-namespace {
-
-constexpr uint32_t COLLISION_CEILING = 1u << 20;  // guards a non-terminating walk only
-
-enum : uint32_t { ST_IDLE = 0, ST_ACTIVE = 1, ST_DEAD = 2 };
-enum : uint32_t { MODE_PRIMARY = 0, MODE_SHADOW = 1 };
-
-struct Lane {
-    uint32_t job;  // batched launch: index into the job table
-    int32_t px, py;
-    uint32_t id;
-    uint32_t f;
-    f4 acc;
-    uint32_t rng;
-    f4 res;
-    float thr;
-    uint32_t bounces;
-    f3 pos, dir;
-    float t, dist;
-    uint32_t mode;
-    float T;      // shadow-ray transmittance
-    float ipdf;   // lightIntensity * phase pdf of the pending shadow ray
-    float w;      // thr * albedo of the pending shadow ray
-    // brick DDA
-    int32_t cell[3];
-    float tNext[3];
-    float tau;
-};
-
-struct Counters {
-    uint32_t paths, vertices, lookups, svLookups, brickVisits;
-};
-
-__device__ __forceinline__ const float* B0(const cr::PTArgs& a) { return a.cloud.bounds[0]; }
-__device__ __forceinline__ const float* B1(const cr::PTArgs& a) { return a.cloud.bounds[1]; }
-
-template <typename T, int LAYOUT>
-__device__ __forceinline__ float sample_cloud(const cr::PTArgs& a, f3 p)
-{
-    GridView g;
-    g.data = a.grid; g.nx = a.nx; g.ny = a.ny; g.nz = a.nz; g.nbx = a.nbx; g.nby = a.nby;
-    return trilinear<T, LAYOUT>(g, cloud_norm(a.cloud, p));
-}
-
-__device__ __forceinline__ float sample_sv(const cr::PTArgs& a, f3 p)
-{
-    GridView g;
-    g.data = a.sv; g.nx = a.svx; g.ny = a.svy; g.nz = a.svz; g.nbx = 0; g.nby = 0;
-    return trilinear<float, LAYOUT_DENSE>(g, sv_norm(a.svp, p));
-}
-
-__device__ __forceinline__ void dda_init(const cr::PTArgs& a, f3 pos, f3 dir, Lane& L)
-{
-    const float p[3] = {pos.x, pos.y, pos.z};
-    const float v[3] = {dir.x, dir.y, dir.z};
-    const int32_t mc[3] = {a.mx, a.my, a.mz};
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        float rel = fdiv(p[k] - B0(a)[k], a.bw[k]);
-        int32_t c = (int32_t)floorf(fminf(fmaxf(rel, 0.0f), (float)mc[k]));
-        c = clampi(c, 0, mc[k] - 1);
-        L.cell[k] = c;
-        if (v[k] > 0.0f) {
-            L.tNext[k] = fmaxf(fdiv((B0(a)[k] + (float)(c + 1) * a.bw[k]) - p[k], v[k]), 0.0f);
-        } else if (v[k] < 0.0f) {
-            L.tNext[k] = fmaxf(fdiv((B0(a)[k] + (float)c * a.bw[k]) - p[k], v[k]), 0.0f);
-        } else {
-            L.tNext[k] = __builtin_inff();
-        }
-    }
-}
-
-template <int K>
-__device__ __forceinline__ void dda_step_axis(const cr::PTArgs& a, float v, int32_t m, Lane& L)
-{
-    int32_t c = L.cell[K] + (v > 0.0f ? 1 : (v < 0.0f ? -1 : 0));
-    L.cell[K] = c;
-    float d;
-    if (v > 0.0f) d = fdiv(a.bw[K], v);
-    else if (v < 0.0f) d = fdiv(a.bw[K], -v);
-    else d = __builtin_inff();
-    L.tNext[K] = L.tNext[K] + d;
-}
-
-// Advance to the neighbouring brick along the axis of the nearest boundary
-// (ties: x before y before z); false when the ray leaves the brick grid.
-__device__ __forceinline__ bool dda_advance(const cr::PTArgs& a, f3 dir, Lane& L)
-{
-    if (L.tNext[0] <= L.tNext[1] && L.tNext[0] <= L.tNext[2]) {
-        dda_step_axis<0>(a, dir.x, a.mx, L);
-        return L.cell[0] >= 0 && L.cell[0] < a.mx;
-    }
-    if (L.tNext[1] <= L.tNext[2]) {
-        dda_step_axis<1>(a, dir.y, a.my, L);
-        return L.cell[1] >= 0 && L.cell[1] < a.my;
-    }
-    dda_step_axis<2>(a, dir.z, a.mz, L);
-    return L.cell[2] >= 0 && L.cell[2] < a.mz;
-}
-
-__device__ __forceinline__ float brick_majorant(const cr::PTArgs& a, const Lane& L)
-{
-    return __ldg(a.majorants + ((uint64_t)L.cell[0] + (uint64_t)a.mx * ((uint64_t)L.cell[1] + (uint64_t)a.my * (uint64_t)L.cell[2])));
-}
-
-// Start a tracking segment from L.pos along `dir` (primary or shadow).
-// Returns false when the segment misses the box (no RNG draw is made then,
-// as findScatterPoint returns before tracking, PathTracer.comp:321-324).
-template <int TRACK>
-__device__ __forceinline__ bool begin_segment(const cr::PTArgs& a, Lane& L, f3 dir)
-{
-    float tmax = 0.0f, tmin = 0.0f;
-    if (!intersect_box(B0(a), B1(a), L.pos, dir, tmin, tmax)) return false;
-    f3 exitp = add(L.pos, mul(dir, tmax));
-    L.dist = length(sub(L.pos, exitp));
-    L.t = 0.0f;
-    if (TRACK == CR_TRACK_BRICK_DDA) {
-        dda_init(a, L.pos, dir, L);
-        L.tau = -crd_logf(xs_float(L.rng));
-    }
-    return true;
-}
-
-enum : uint32_t { EV_NONE = 0, EV_COLLIDE = 1, EV_EXIT = 2 };
-
-// One tracking step: at most one density lookup.  Primary mode: delta
-// tracking (PathTracer.comp:335-355); shadow mode: ratio tracking.
-template <typename T, int LAYOUT, int TRACK>
-__device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 sdir, Counters& c)
-{
-    const bool shadow = L.mode == MODE_SHADOW;
-    const f3 dir = shadow ? sdir : L.dir;
-    if (TRACK == CR_TRACK_BRICK_DDA) {
-        float maj;
-        for (;;) {
-            maj = brick_majorant(a, L);
-            float sb = fdiv(maj * a.cloud.densityScaling, a.cloud.baseScaling);
-            float tc = fminf(fminf(fminf(L.tNext[0], L.tNext[1]), L.tNext[2]), L.dist);
-            float seg = tc - L.t;
-            c.brickVisits++;
-            if (sb > 0.0f && L.tau < sb * seg) {
-                L.t = L.t + fdiv(L.tau, sb);
-                break;
-            }
-            L.tau = L.tau - sb * seg;
-            L.t = tc;
-            if (!(tc < L.dist)) return EV_EXIT;
-            if (!dda_advance(a, dir, L)) return EV_EXIT;
-        }
-        f3 cur = add(L.pos, mul(dir, L.t));
-        float e = sample_cloud<T, LAYOUT>(a, cur);
-        c.lookups++;
-        if (shadow) {
-            L.T = L.T * (1.0f - fdiv(e, maj));
-            if (L.T == 0.0f) return EV_EXIT;
-            L.tau = -crd_logf(xs_float(L.rng));
-            return EV_NONE;
-        }
-        float xi = xs_float(L.rng);
-        if (xi < fdiv(e, maj)) {
-            L.pos = cur;
-            return EV_COLLIDE;
-        }
-        L.tau = -crd_logf(xs_float(L.rng));
-        return EV_NONE;
-    }
-    float zeta = xs_float(L.rng);
-    L.t = L.t + fdiv(-crd_logf(zeta), a.sigmaBar);
-    if (!(L.t < L.dist)) return EV_EXIT;
-    f3 cur = add(L.pos, mul(dir, L.t));
-    float e = sample_cloud<T, LAYOUT>(a, cur);
-    c.lookups++;
-    if (shadow) {
-        L.T = L.T * (1.0f - fdiv(e, a.cloud.maxExtinction));
-        if (L.T == 0.0f) return EV_EXIT;
-        return EV_NONE;
-    }
-    float xi = xs_float(L.rng);
-    if (xi < fdiv(e, a.cloud.maxExtinction)) {
-        L.pos = cur;
-        return EV_COLLIDE;
-    }
-    return EV_NONE;
-}
-
-__device__ __forceinline__ void add_scaled(f4& r, const f4& v, float s)
-{
-    r.x = r.x + v.x * s;
-    r.y = r.y + v.y * s;
-    r.z = r.z + v.z * s;
-    r.w = r.w + v.w * s;
-}
-
-__device__ __forceinline__ void add_scalar(f4& r, float s)
-{
-    r.x = r.x + s;
-    r.y = r.y + s;
-    r.z = r.z + s;
-    r.w = r.w + s;
-}
-
-// Blend the finished sample into the running mean (PathTracer.comp:420-424).
-__device__ __forceinline__ void blend(const cr::PTArgs& a, Lane& L)
-{
-    const uint32_t fc = (uint32_t)a.frames[2 * L.f + 1];
-    const float n = (float)fc;
-    const float d = (float)(fc + 1u);
-    L.acc.x = fdiv(L.res.x + L.acc.x * n, d);
-    L.acc.y = fdiv(L.res.y + L.acc.y * n, d);
-    L.acc.z = fdiv(L.res.z + L.acc.z * n, d);
-    L.acc.w = fdiv(L.res.w + L.acc.w * n, d);
-}
-
-// The job table of a batched launch lives in dynamic LDS.
-template <bool BATCH> __device__ __forceinline__ cr::PTArgs* job_table();
-template <> __device__ __forceinline__ cr::PTArgs* job_table<false>() { return nullptr; }
-template <> __device__ __forceinline__ cr::PTArgs* job_table<true>()
-{
-    extern __shared__ uint64_t pt_job_lds[];
-    return reinterpret_cast<cr::PTArgs*>(pt_job_lds);
-}
-
-// Last job whose first work index is <= idx (jobs are stored by increasing base).
-__device__ __forceinline__ uint32_t find_job(const cr::PTArgs* jobs, uint32_t n, uint32_t idx)
-{
-    uint32_t lo = 0, hi = n - 1u;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1u) >> 1;
-        if (jobs[mid].base <= idx) lo = mid;
-        else hi = mid - 1u;
-    }
-    return lo;
-}
-
-// BATCH = false: one job, `a0`.  BATCH = true: the jobs of `b`, one shared
-// work queue over the concatenated pixel ranges; every lane follows its own
-// job's arguments (read from LDS), so lanes of one wave may render different
-// jobs.  Per pixel the arithmetic is the same in both forms.
-template <typename T, int LAYOUT, int TRACK, int NEE, bool BATCH>
-__global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
-{
-    cr::PTArgs* const jobs = job_table<BATCH>();
-    if (BATCH) {
-        const uint32_t words = b.nJobs * (uint32_t)(sizeof(cr::PTArgs) / 4);
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(b.jobs);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(jobs);
-        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
-        __syncthreads();
-    }
-    uint32_t* const queue = BATCH ? b.queue : a0.queue;
-    const uint32_t totalIdx = BATCH ? b.total : a0.totalIdx;
-    Lane L;
-    Counters c = {0, 0, 0, 0, 0};
-    uint32_t state = ST_IDLE;
-    const uint32_t lane = lane_id();
-    const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    L.mode = MODE_PRIMARY;
-    L.job = 0;
-
-    for (;;) {
-        // ---- claim pixels for idle lanes (wave-wide, one atomic per wave)
-        const unsigned long long idle = __ballot(state == ST_IDLE);
-        if (idle) {
-            uint32_t base = 0;
-            const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
-            if (lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(idle));
-            base = __shfl(base, (int)leader, 64);
-            if (state == ST_IDLE) {
-                const uint32_t idx = base + (uint32_t)__popcll(idle & lt_mask);
-                if (idx >= totalIdx) {
-                    state = ST_DEAD;
-                } else {
-                    const uint32_t j = BATCH ? find_job(jobs, b.nJobs, idx) : 0u;
-                    const cr::PTArgs& J = BATCH ? jobs[j] : a0;
-                    const uint32_t local = BATCH ? idx - J.base : idx;
-                    const uint32_t tile = local >> 6, within = local & 63u;
-                    const uint32_t tl = J.tileLog2W, tw = 1u << tl;
-                    const int32_t x = (int32_t)((tile % J.tilesX) * tw + (within & (tw - 1u)));
-                    const int32_t y = (int32_t)((tile / J.tilesX) * (64u >> tl) + (within >> tl));
-                    if (x < J.W && y < J.H) {
-                        L.job = j;
-                        L.px = x;
-                        L.py = y;
-                        L.id = (uint32_t)x + (uint32_t)y * J.padded;
-                        float4 v = J.image[(size_t)y * J.W + x];
-                        L.acc.x = v.x; L.acc.y = v.y; L.acc.z = v.z; L.acc.w = v.w;
-                        L.f = 0;
-                        state = ST_ACTIVE;
-                        // start frame 0: handled by the event loop below
-                        L.mode = 0xFFu;  // marker: needs a new sample
-                    }
-                }
-            }
-        }
-        if (__ballot(state != ST_DEAD) == 0ull) break;
-        if (state != ST_ACTIVE) continue;
-
-        const cr::PTArgs& a = BATCH ? jobs[L.job] : a0;
-        const f3 ldir = ld3(a.svp.lightDirection);
-        const f3 sdir = neg(ldir);
-        const float I = a.params.lightIntensity;
-        const float albedo = a.ext.albedo;
-        const uint32_t cap = a.ext.bounceCap;
-
-        uint32_t ev = EV_NONE;
-        if (L.mode != 0xFFu) ev = track_step<T, LAYOUT, TRACK>(a, L, sdir, c);
-
-        // ---- divergent tail: events that need no lookup are resolved here,
-        // possibly several in a row (e.g. a camera ray that misses the box).
-        bool needSample = (L.mode == 0xFFu);
-        while (ev != EV_NONE || needSample) {
-            if (needSample) {
-                needSample = false;
-                if (L.f >= a.nFrames) {
-                    float4 o;
-                    o.x = L.acc.x; o.y = L.acc.y; o.z = L.acc.z; o.w = L.acc.w;
-                    a.image[(size_t)L.py * a.W + L.px] = o;
-                    state = ST_IDLE;
-                    break;
-                }
-                // start frame f: PathTracer.comp:368-392
-                const int32_t seed = a.frames[2 * L.f];
-                xs_init(L.rng, (uint32_t)seed * L.id);
-                camera_ray(a.cam, (float)(L.px - a.cam.halfWidth), (float)(L.py - a.cam.halfHeight), L.pos, L.dir);
-                c.paths++;
-                c.vertices++;
-                L.res.x = 0.0f; L.res.y = 0.0f; L.res.z = 0.0f; L.res.w = 0.0f;
-                L.thr = 1.0f;
-                L.bounces = 0;
-                L.mode = MODE_PRIMARY;
-                float tmax = 0.0f, tmin = 0.0f;
-                if (!intersect_box(B0(a), B1(a), L.pos, L.dir, tmin, tmax) || tmax < 0.0f ||
-                    a.cloud.densityScaling <= 0.0f) {
-                    L.res = background(a.ext, L.dir);
-                    blend(a, L);
-                    L.f++;
-                    needSample = true;
-                    continue;
-                }
-                L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
-                if (!begin_segment<TRACK>(a, L, L.dir)) ev = EV_EXIT;
-                else break;
-            }
-            if (ev == EV_EXIT && L.mode == MODE_PRIMARY) {
-                // escape: PathTracer.comp:399-403
-                add_scaled(L.res, background(a.ext, L.dir), L.thr);
-                blend(a, L);
-                L.f++;
-                ev = EV_NONE;
-                needSample = true;
-                continue;
-            }
-            if (ev == EV_EXIT && L.mode == MODE_SHADOW) {
-                // shadow ray done: add NEE and scatter
-                add_scalar(L.res, ((L.ipdf * L.T) * L.w));
-                L.dir = scatter(a.params, L.dir, L.rng);
-                L.thr = L.w;
-                L.mode = MODE_PRIMARY;
-                ev = begin_segment<TRACK>(a, L, L.dir) ? EV_NONE : EV_EXIT;
-                continue;
-            }
-            if (ev == EV_COLLIDE) {
-                L.bounces++;
-                if (L.bounces > cap || L.bounces > COLLISION_CEILING) {
-                    blend(a, L);
-                    L.f++;
-                    ev = EV_NONE;
-                    needSample = true;
-                    continue;
-                }
-                c.vertices++;
-                const float pdf = phase_pdf(a.params, neg(L.dir), ldir);
-                const float w = L.thr * albedo;
-                if (NEE == CR_NEE_RATIO_TRACKING) {
-                    L.ipdf = (1.0f * I) * pdf;
-                    L.w = w;
-                    L.T = 1.0f;
-                    L.mode = MODE_SHADOW;
-                    // a shadow ray that misses the box keeps T = 1 and draws nothing
-                    ev = begin_segment<TRACK>(a, L, sdir) ? EV_NONE : EV_EXIT;
-                    continue;
-                }
-                float Tv;
-                if (NEE == CR_NEE_SHADOW_VOLUME) {
-                    Tv = sample_sv(a, L.pos);
-                    c.svLookups++;
-                } else {
-                    Tv = 0.0f;
-                }
-                add_scalar(L.res, (((1.0f * I) * pdf) * Tv) * w);
-                L.dir = scatter(a.params, L.dir, L.rng);
-                L.thr = w;
-                ev = begin_segment<TRACK>(a, L, L.dir) ? EV_NONE : EV_EXIT;
-                continue;
-            }
-        }
-    }
-
-    // ---- counters: one atomic per wave (all lanes have reconverged here)
-    unsigned long long s0 = wave_sum_u64(c.paths), s1 = wave_sum_u64(c.vertices), s2 = wave_sum_u64(c.lookups);
-    unsigned long long s3 = wave_sum_u64(c.svLookups), s4 = wave_sum_u64(c.brickVisits);
-    if (lane == 0) {
-        unsigned long long* const cnt = BATCH ? b.counters : a0.counters;
-        atomicAdd(cnt + 0, s0);
-        atomicAdd(cnt + 1, s1);
-        atomicAdd(cnt + 2, s2);
-        atomicAdd(cnt + 3, s3);
-        atomicAdd(cnt + 4, s4);
-    }
-}
-
-using PtFn = void (*)(cr::PTArgs, cr::BatchArgs);
-
-template <typename T, int LAYOUT, bool BATCH>
-PtFn pick_tn(uint32_t tr, uint32_t ne)
-{
-    if (tr == CR_TRACK_BRICK_DDA) {
-        if (ne == CR_NEE_RATIO_TRACKING) return pt_kernel<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_RATIO_TRACKING, BATCH>;
-        if (ne == CR_NEE_NONE) return pt_kernel<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_NONE, BATCH>;
-        return pt_kernel<T, LAYOUT, CR_TRACK_BRICK_DDA, CR_NEE_SHADOW_VOLUME, BATCH>;
-    }
-    if (ne == CR_NEE_RATIO_TRACKING) return pt_kernel<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_RATIO_TRACKING, BATCH>;
-    if (ne == CR_NEE_NONE) return pt_kernel<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_NONE, BATCH>;
-    return pt_kernel<T, LAYOUT, CR_TRACK_GLOBAL_MAJORANT, CR_NEE_SHADOW_VOLUME, BATCH>;
-}
-
-// The kernel instantiation for the grid storage and the render modes.
-template <bool BATCH>
-PtFn pick(int dtype, int layout, uint32_t tr, uint32_t ne)
-{
-    if (dtype == CR_DTYPE_F16) {
-        if (layout == LAYOUT_BRICK4) return pick_tn<__half, LAYOUT_BRICK4, BATCH>(tr, ne);
-        return pick_tn<__half, LAYOUT_DENSE, BATCH>(tr, ne);
-    }
-    if (layout == LAYOUT_BRICK4) return pick_tn<float, LAYOUT_BRICK4, BATCH>(tr, ne);
-    return pick_tn<float, LAYOUT_DENSE, BATCH>(tr, ne);
-}
-
-hipError_t launch(PtFn fn, cr::PTArgs a0, cr::BatchArgs b, int blocks, int threads, size_t lds, hipStream_t s)
-{
-    void* args[] = {&a0, &b};
-    return hipLaunchKernel(reinterpret_cast<const void*>(fn), dim3((unsigned)blocks), dim3((unsigned)threads), args, lds, s);
-}
-
-int occupancy(PtFn fn, int threads, size_t lds)
-{
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, lds) != hipSuccess) return 1;
-    return n > 0 ? n : 1;
-}
-
-} // namespace
+// Launches of the path tracer: the single-job render and the batched render.
+#include "pt_kernel.h"
 
 namespace cr {
 
 hipError_t launch_pt(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s)
 {
     BatchArgs none = {};
-    return launch(pick<false>(dtype, layout, a.ext.trackingMode, a.ext.neeMode), a, none, blocks, 256, 0, s);
+    return launch(pick<false, K_RENDER>(dtype, layout, a.ext.trackingMode, a.ext.neeMode), a, none, blocks, 256, 0, s);
 }
 
 int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee)
 {
-    return occupancy(pick<false>(dtype, layout, tracking, nee), 256, 0);
+    return occupancy(pick<false, K_RENDER>(dtype, layout, tracking, nee), 256, 0);
 }
 
 hipError_t launch_pt_batch(const BatchArgs& b, int dtype, int layout, uint32_t tracking, uint32_t nee, int blocks,
                            int block_threads, hipStream_t s)
 {
     PTArgs none = {};
-    return launch(pick<true>(dtype, layout, tracking, nee), none, b, blocks, block_threads,
+    return launch(pick<true, K_RENDER>(dtype, layout, tracking, nee), none, b, blocks, block_threads,
                   (size_t)b.nJobs * sizeof(PTArgs), s);
 }
 
 int pt_batch_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, int block_threads, uint32_t n_jobs)
 {
-    return occupancy(pick<true>(dtype, layout, tracking, nee), block_threads, (size_t)n_jobs * sizeof(PTArgs));
+    return occupancy(pick<true, K_RENDER>(dtype, layout, tracking, nee), block_threads, (size_t)n_jobs * sizeof(PTArgs));
 }
 
 } // namespace cr

@@ -365,6 +365,37 @@ CR_API cr_status cr_learn_build_mips(cr_context* ctx);
  * the current cloud properties and shadow-volume light direction. */
 CR_API cr_status cr_learn_features(cr_context* ctx, const float* points, uint32_t n, uint16_t* X);
 
+/* Record pass: each frame's camera path (cr_pt_render's sample) up to its
+ * first real collision.  records receives [n_frames][H][W] records of 8
+ * floats: a collision is (x, y, z, dx, dy, dz, RNG state bits, 1); a path
+ * that escapes before any collision is (r, g, b, a, 0, 0, 0, 0) with its
+ * radiance.  The accumulation image is not touched. */
+CR_API cr_status cr_pt_record(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames, float* records);
+/* Continuation pass: for each of n records, the mean radiance (out: [n][4])
+ * of spp continuations of the path from its collision (NEE, scattering,
+ * further bounces, exactly as cr_pt_render continues it).  The first
+ * continuation resumes the recorded RNG state, so record + continue with
+ * spp = 1 reproduces cr_pt_render's sample bit for bit; escaped records give 0. */
+CR_API cr_status cr_pt_continue(cr_context* ctx, const float* records, uint32_t n, uint32_t spp, float* out);
+/* Training data from the current scene: record pass over `frames`, the
+ * collisions compacted in record order, every k-th of them (k chosen so that
+ * at most max_points remain; 0 = all) continued with spp samples (the target)
+ * and featurised; appended to the library's data set.  *added = rows added. */
+CR_API cr_status cr_learn_collect(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames, uint32_t spp,
+                                  uint32_t max_points, uint32_t* added);
+CR_API cr_status cr_learn_dataset_size(cr_context* ctx, uint64_t* n);
+CR_API cr_status cr_learn_dataset_clear(cr_context* ctx);
+/* Rows [first, first+n) of the data set: X [n][224] bf16, T [n][3]; either may be NULL. */
+CR_API cr_status cr_learn_dataset_download(cr_context* ctx, uint64_t first, uint32_t n, uint16_t* X, float* T);
+/* steps Adam steps on minibatches of `batch` rows drawn from the data set by a
+ * seeded hash; initialises the network (cr_mlp_init(seed)) if there is none.
+ * *loss = the last step's loss. */
+CR_API cr_status cr_learn_train(cr_context* ctx, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* loss);
+/* Learned render: record pass, features and network at every collision, and
+ * the running mean into the accumulation image (cr_pt_render's contract, with
+ * the network standing for the rest of each path). */
+CR_API cr_status cr_learn_render(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames);
+
 #ifdef __cplusplus
 }
 #endif
