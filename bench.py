@@ -38,7 +38,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c5"],
+                    help="c1/c2: plain path tracer throughput; c5: learned vs plain time to relMSE <= 1e-3")
+    ap.add_argument("--c5-clouds", type=int, default=200)
+    ap.add_argument("--c5-points", type=int, default=80000, help="training points per cloud")
+    ap.add_argument("--c5-target-spp", type=int, default=1024, help="continuations per training point")
+    ap.add_argument("--c5-train-steps", type=int, default=8000)
+    ap.add_argument("--c5-batch", type=int, default=4096)
+    ap.add_argument("--c5-ref-spp", type=int, default=16384, help="frames of the converged plain reference")
+    ap.add_argument("--c5-width", type=int, default=1920)
+    ap.add_argument("--c5-height", type=int, default=1080)
     ap.add_argument("--spp-per-step", type=int, default=0, help="frames per step; 0 = the workload's full spp")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -62,6 +71,8 @@ def main():
     pkg = cr_import.package()
     cfg = cr_import.configs()
     S = pkg.structs
+    if args.workload == "c5":
+        return bench_c5(args, pkg, cfg)
 
     sc = cfg.c2() if args.workload == "c2" else cfg.c1()
     ctx = pkg.Context(0)
@@ -171,6 +182,127 @@ def main():
             "Mpaths_per_s": round(st.paths / t_total / 1e6, 3),
             "spp_per_s_1080p": round(st.paths / t_total / (W * H), 3),
         },
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def rel_mse(img, ref, eps=1e-2):
+    """mean over pixels and RGB of (I - R)^2 / (R^2 + eps)."""
+    import numpy as np
+    a = img[..., :3].astype(np.float64)
+    r = ref[..., :3].astype(np.float64)
+    return float(np.mean((a - r) ** 2 / (r * r + eps)))
+
+
+def bench_c5(args, pkg, cfg):
+    """Learned in-scattering vs the plain tracer: train on samples the plain
+    tracer generates over seeded 128^3 clouds of c1 statistics, then time both
+    renderers at 1080p on a held-out cloud until relMSE <= 1e-3 against a
+    converged plain render."""
+    import math
+    import numpy as np
+    ctx = pkg.Context(0)
+    rng = np.random.default_rng(5)
+
+    def cloud_scene(seed, light, w=256, h=256):
+        sc = cfg.c1(width=w, height=h)
+        sc.fbm.seed = seed
+        sc.light_dir = light
+        return sc
+
+    def sun():
+        ct = float(rng.uniform(0.2, 1.0))
+        ph = float(rng.uniform(0.0, 2.0 * math.pi))
+        st = math.sqrt(1.0 - ct * ct)
+        return (st * math.cos(ph), -ct, st * math.sin(ph))
+
+    # ---- training data from the plain tracer
+    t0 = time.perf_counter()
+    ctx.learn_dataset_clear()
+    for k in range(args.c5_clouds):
+        sc = cloud_scene(1000 + k, sun())
+        sc.setup(ctx)
+        ctx.learn_collect(cfg.frames(3, srand_seed=1 + k), args.c5_target_spp, args.c5_points)
+        if (k + 1) % 20 == 0:
+            log(f"c5: {k + 1} clouds collected, {ctx.learn_dataset_size()} points, {time.perf_counter() - t0:.1f}s")
+    ctx.synchronize()
+    t_collect = time.perf_counter() - t0
+    n_points = ctx.learn_dataset_size()
+    t0 = time.perf_counter()
+    ctx.mlp_init(11)
+    loss = None
+    for s in range(0, args.c5_train_steps, 1000):
+        loss = ctx.learn_train(min(1000, args.c5_train_steps - s), args.c5_batch, 1e-3 if s < args.c5_train_steps // 2 else 3e-4, seed=3)
+        log(f"c5: {s + 1000} training steps, loss {loss:.4g}")
+    ctx.synchronize()
+    t_train = time.perf_counter() - t0
+
+    # ---- held-out cloud at 1080p
+    W, H = args.c5_width, args.c5_height
+    sc = cloud_scene(7, (1.0, -1.0, 0.0), W, H)
+    sc.setup(ctx)
+    t0 = time.perf_counter()
+    ref_frames = cfg.frames(args.c5_ref_spp, srand_seed=99)
+    for i in range(0, args.c5_ref_spp, 256):
+        ctx.pt_render(ref_frames[i:i + 256])
+        ctx.synchronize()
+    ref = ctx.download_image(H, W)
+    t_ref = time.perf_counter() - t0
+    log(f"c5: reference {args.c5_ref_spp} spp in {t_ref:.1f}s")
+
+    def progressive(render, max_spp, chunk):
+        """(spp, seconds, relMSE) after each step of a 1, 2, 3, 4, 6, 8, 11, ... schedule;
+        stops at relMSE <= 1e-3, or when the error has stopped falling."""
+        ctx.clear_image()
+        ctx.synchronize()
+        out, done, secs = [], 0, 0.0
+        frames = cfg.frames(max_spp)
+        sched = sorted({int(round(2 ** (i / 2.0))) for i in range(0, 64)})
+        for target in sched:
+            if target > max_spp:
+                break
+            t = time.perf_counter()
+            for i in range(done, target, chunk):
+                render(frames[i:min(target, i + chunk)])
+            ctx.synchronize()
+            secs += time.perf_counter() - t
+            done = target
+            e = rel_mse(ctx.download_image(H, W), ref)
+            out.append((done, secs, e))
+            if e <= 1e-3:
+                break
+            if len(out) >= 5 and e > 0.97 * out[-3][2]:
+                break  # a floor: the estimator's bias, not its noise
+        return out
+
+    plain = progressive(ctx.pt_render, 4096, 256)
+    learned = progressive(ctx.learn_render, 1024, 8)
+
+    def first_below(curve):
+        for spp, secs, e in curve:
+            if e <= 1e-3:
+                return spp, secs
+        return None, None
+
+    p_spp, p_s = first_below(plain)
+    l_spp, l_s = first_below(learned)
+    out = {
+        "metric": METRIC,
+        "value": round(p_s / l_s, 3) if (p_s and l_s) else None,
+        "unit": "x (plain time / learned time to relMSE <= 1e-3)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16 network, f32 tracer",
+        "data": "synthetic",
+        "config": {"workload": f"c5: learned in-scattering, {args.c5_clouds} seeded 128^3 clouds of c1 statistics, "
+                               f"{args.c5_target_spp} continuations per training point, evaluation {W}x{H} on a held-out "
+                               f"cloud against a {args.c5_ref_spp}-spp plain render"},
+        "plain": {"spp": p_spp, "seconds": p_s, "curve": [[a, round(b, 4), round(c, 6)] for a, b, c in plain]},
+        "learned": {"spp": l_spp, "seconds": l_s, "curve": [[a, round(b, 4), round(c, 6)] for a, b, c in learned]},
+        "training": {"points": n_points, "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
+                     "steps": args.c5_train_steps, "batch": args.c5_batch, "final_loss": loss},
+        "reference_s": round(t_ref, 2),
     }
     ctx.close()
     print(json.dumps(out), flush=True)

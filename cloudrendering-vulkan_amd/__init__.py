@@ -120,6 +120,13 @@ class Context:
         return out
 
     # ------------------------------------------------------------ uniforms
+    def skip_grid(self):
+        d = (C.c_uint32 * 3)()
+        self.check(self.lib.cr_grid_majorants(self.h, d, None), "cr_grid_majorants")
+        out = np.empty((d[2], d[1], d[0]), np.float32)
+        self.check(self.lib.cr_grid_skip(self.h, _ptr(out)), "cr_grid_skip")
+        return out
+
     def set_camera(self, cam):
         self.check(self.lib.cr_set_camera(self.h, C.byref(cam)), "cr_set_camera")
 

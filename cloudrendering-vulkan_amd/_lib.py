@@ -43,6 +43,7 @@ SIGNATURES = {
     "cr_grid_save_xyz": (st, [ctx_p, C.c_char_p]),
     "cr_grid_download": (st, [ctx_p, vp]),
     "cr_grid_majorants": (st, [ctx_p, P(C.c_uint32), vp]),
+    "cr_grid_skip": (st, [ctx_p, vp]),
     "cr_fbm_params_default": (None, [P(S.FbmParams)]),
     "cr_grid_generate_fbm": (st, [ctx_p, P(S.FbmParams), C.c_int]),
     "cr_set_camera": (st, [ctx_p, P(S.CameraProperties)]),

@@ -134,7 +134,7 @@ cr_status cr_destroy(cr_context* ctx)
     CR_CHECK_CTX(ctx);
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    void* bufs[] = {ctx->grid.data, ctx->grid.majorants, ctx->sv, ctx->image, ctx->dFrames, ctx->dQueue,
+    void* bufs[] = {ctx->grid.data, ctx->grid.majorants, ctx->grid.skip, ctx->sv, ctx->image, ctx->dFrames, ctx->dQueue,
                     ctx->dCounters, ctx->dScalar};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -218,20 +218,32 @@ int majorant_brick(const cr_render_ext& e) { return e.majorantBrick ? (int)e.maj
 cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb)
 {
     int32_t mx = (g.nx + mb - 1) / mb, my = (g.ny + mb - 1) / mb, mz = (g.nz + mb - 1) / mb;
+    const size_t n = (size_t)mx * my * mz;
     if (g.majorants && (g.mx != mx || g.my != my || g.mz != mz)) {
         hipFree(g.majorants);
+        hipFree(g.skip);
         g.majorants = nullptr;
+        g.skip = nullptr;
     }
     if (!g.majorants) {
-        hipError_t e = hipMalloc((void**)&g.majorants, (size_t)mx * my * mz * sizeof(float));
+        hipError_t e = hipMalloc((void**)&g.majorants, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void**)&g.skip, n * sizeof(float));
         if (e != hipSuccess) {
+            if (g.majorants) hipFree(g.majorants);
             g.majorants = nullptr;
+            g.skip = nullptr;
             return fail(ctx, CR_ERR_OUT_OF_MEMORY, "hipMalloc majorants");
         }
     }
     g.mx = mx; g.my = my; g.mz = mz; g.mb = mb;
     CR_HIP(ctx, launch_majorants(g.data, g.dtype, g.layout, g.nx, g.ny, g.nz, g.nbx, g.nby, mb, g.majorants, mx, my, mz,
                                  ctx->stream), "majorant kernel");
+    uint8_t* tmp = nullptr;
+    CR_HIP(ctx, hipMallocAsync((void**)&tmp, 2 * n, ctx->stream), "skip scratch");
+    hipError_t e = launch_skip_grid(g.majorants, mx, my, mz, g.skip, tmp, ctx->stream);
+    hipError_t e2 = hipFreeAsync(tmp, ctx->stream);
+    CR_HIP(ctx, e, "skip grid");
+    CR_HIP(ctx, e2, "free skip scratch");
     return CR_OK;
 }
 
@@ -353,6 +365,17 @@ cr_status cr_grid_majorants(cr_context* ctx, uint32_t dims[3], float* out)
         CR_HIP(ctx, hipMemcpyAsync(out, ctx->grid.majorants, n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy majorants");
         CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     }
+    return CR_OK;
+}
+
+cr_status cr_grid_skip(cr_context* ctx, float* out)
+{
+    CR_CHECK_CTX(ctx);
+    if (!ctx->hasGrid) return fail(ctx, CR_ERR_STATE, "no grid");
+    if (!out) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null output");
+    size_t n = (size_t)ctx->grid.mx * ctx->grid.my * ctx->grid.mz;
+    CR_HIP(ctx, hipMemcpyAsync(out, ctx->grid.skip, n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy skip grid");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return CR_OK;
 }
 
@@ -632,7 +655,7 @@ void fill_job_args(PTArgs& a, const DeviceGrid& g, const float* sv, const uint32
 {
     memset(&a, 0, sizeof(a));
     a.grid = g.data; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nbx = g.nbx; a.nby = g.nby;
-    a.majorants = g.majorants; a.mx = g.mx; a.my = g.my; a.mz = g.mz;
+    a.majorants = g.majorants; a.skip = g.skip; a.mx = g.mx; a.my = g.my; a.mz = g.mz;
     const int32_t dims[3] = {g.nx, g.ny, g.nz};
     for (int k = 0; k < 3; k++) a.bw[k] = ((cloud.bounds[1][k] - cloud.bounds[0][k]) * (float)g.mb) / (float)dims[k];
     a.sv = sv;

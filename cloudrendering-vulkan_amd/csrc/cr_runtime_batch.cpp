@@ -70,7 +70,7 @@ namespace {
 
 void free_job(BatchJob& j)
 {
-    void* bufs[] = {j.grid.data, j.grid.majorants, j.sv, j.image, j.dFrames};
+    void* bufs[] = {j.grid.data, j.grid.majorants, j.grid.skip, j.sv, j.image, j.dFrames};
     for (void* b : bufs)
         if (b) hipFree(b);
     j = BatchJob();

@@ -176,6 +176,36 @@ __global__ void __launch_bounds__(256) majorant_kernel(const T* g, int32_t nx, i
     if (threadIdx.x == 0) out[(uint64_t)bx + (uint64_t)mx * ((uint64_t)by + (uint64_t)my * (uint64_t)bz)] = red[0];
 }
 
+// Skip grid: occupancy, then 15 rounds of a 3x3x3 dilation (one axis per
+// pass); a brick first covered in round d is d bricks (Chebyshev) from the
+// nearest non-empty brick.  Bricks still uncovered keep -16.
+__global__ void __launch_bounds__(256) skip_init_kernel(const float* maj, uint32_t n, uint8_t* occ, float* skip)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float m = maj[i];
+    occ[i] = m > 0.0f ? 1 : 0;
+    skip[i] = m > 0.0f ? m : -16.0f;
+}
+
+__global__ void __launch_bounds__(256) dilate_kernel(const uint8_t* in, uint8_t* out, int32_t mx, int32_t my, int32_t mz,
+                                                     int axis, float* skip, float d)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t n = (uint32_t)mx * (uint32_t)my * (uint32_t)mz;
+    if (i >= n) return;
+    const int32_t x = (int32_t)(i % (uint32_t)mx), y = (int32_t)((i / (uint32_t)mx) % (uint32_t)my);
+    const int32_t z = (int32_t)(i / ((uint32_t)mx * (uint32_t)my));
+    const int32_t c = axis == 0 ? x : (axis == 1 ? y : z);
+    const int32_t N = axis == 0 ? mx : (axis == 1 ? my : mz);
+    const uint32_t stride = axis == 0 ? 1u : (axis == 1 ? (uint32_t)mx : (uint32_t)mx * (uint32_t)my);
+    uint8_t v = in[i];
+    if (c > 0) v |= in[i - stride];
+    if (c + 1 < N) v |= in[i + stride];
+    out[i] = v;
+    if (skip && v && skip[i] == -16.0f) skip[i] = -d;  // last pass of round d
+}
+
 // Majorants are >= 0, so their IEEE bit patterns order like unsigned integers.
 __global__ void __launch_bounds__(256) max_kernel(const float* m, uint32_t n, unsigned int* out)
 {
@@ -265,6 +295,25 @@ hipError_t launch_math_probe(int op, const float* in, uint32_t n, float* out, hi
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(math_probe_kernel, dim3((n + 255u) / 256u), dim3(256), 0, s, op, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_skip_grid(const float* majorants, int32_t mx, int32_t my, int32_t mz, float* skip, uint8_t* tmp,
+                            hipStream_t s)
+{
+    const uint32_t n = (uint32_t)mx * (uint32_t)my * (uint32_t)mz;
+    const dim3 g((n + 255) / 256);
+    uint8_t* a = tmp;
+    uint8_t* b = tmp + n;
+    hipLaunchKernelGGL(skip_init_kernel, g, dim3(256), 0, s, majorants, n, a, skip);
+    for (int d = 1; d <= 15; d++) {
+        hipLaunchKernelGGL(dilate_kernel, g, dim3(256), 0, s, a, b, mx, my, mz, 0, (float*)nullptr, 0.0f);
+        hipLaunchKernelGGL(dilate_kernel, g, dim3(256), 0, s, b, a, mx, my, mz, 1, (float*)nullptr, 0.0f);
+        hipLaunchKernelGGL(dilate_kernel, g, dim3(256), 0, s, a, b, mx, my, mz, 2, skip, (float)d);
+        uint8_t* t = a;
+        a = b;
+        b = t;
+    }
     return hipGetLastError();
 }
 

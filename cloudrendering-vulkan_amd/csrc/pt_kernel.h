@@ -132,9 +132,11 @@ __device__ __forceinline__ bool dda_advance(const cr::PTArgs& a, f3 dir, Lane& L
     return L.cell[2] >= 0 && L.cell[2] < a.mz;
 }
 
+// Brick majorant, or -d for an empty brick d bricks from the nearest
+// non-empty one (the skip grid).
 __device__ __forceinline__ float brick_majorant(const cr::PTArgs& a, const Lane& L)
 {
-    return __ldg(a.majorants + ((uint64_t)L.cell[0] + (uint64_t)a.mx * ((uint64_t)L.cell[1] + (uint64_t)a.my * (uint64_t)L.cell[2])));
+    return a.skip[(uint32_t)L.cell[0] + (uint32_t)a.mx * ((uint32_t)L.cell[1] + (uint32_t)a.my * (uint32_t)L.cell[2])];
 }
 
 // Start a tracking segment from L.pos along `dir` (primary or shadow).
@@ -166,9 +168,30 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
     const f3 dir = shadow ? sdir : L.dir;
     if (TRACK == CR_TRACK_BRICK_DDA) {
         float maj;
+        // Empty-space skipping: the skip grid holds -d for an empty brick whose
+        // nearest non-empty brick is d bricks away (Chebyshev), so every brick
+        // of the cube of radius d-1 around it is empty; while the walk stays in
+        // that cube no majorant is fetched.  The arithmetic of the walk is
+        // unchanged (an empty brick has majorant 0 either way).
+        int32_t ex = 0, ey = 0, ez = 0, er = -1;
         for (;;) {
-            maj = brick_majorant(a, L);
-            float sb = fdiv(maj * a.cloud.densityScaling, a.cloud.baseScaling);
+            float sb;
+            if (abs(L.cell[0] - ex) <= er && abs(L.cell[1] - ey) <= er && abs(L.cell[2] - ez) <= er) {
+                maj = 0.0f;
+                sb = 0.0f;
+            } else {
+                maj = brick_majorant(a, L);
+                if (maj < 0.0f) {
+                    ex = L.cell[0];
+                    ey = L.cell[1];
+                    ez = L.cell[2];
+                    er = (int32_t)(-maj) - 1;
+                    maj = 0.0f;
+                    sb = 0.0f;
+                } else {
+                    sb = fdiv(maj * a.cloud.densityScaling, a.cloud.baseScaling);
+                }
+            }
             float tc = fminf(fminf(fminf(L.tNext[0], L.tNext[1]), L.tNext[2]), L.dist);
             float seg = tc - L.t;
             c.brickVisits++;

@@ -131,6 +131,10 @@ CR_API cr_status cr_grid_download(cr_context* ctx, float* out);
 /* Brick-majorant grid (extension): ceil(N/b) per axis for b-voxel bricks, x
  * fastest, max density over each b^3 brick plus a one-voxel apron. */
 CR_API cr_status cr_grid_majorants(cr_context* ctx, uint32_t dims_out[3], float* out /* may be NULL */);
+/* The empty-space skip grid the brick walk reads (extension): the majorant of
+ * a non-empty brick, and -d for an empty brick whose nearest non-empty brick
+ * is d bricks away in the max norm (d <= 16; 16 = at least 16). */
+CR_API cr_status cr_grid_skip(cr_context* ctx, float* out);
 
 /* Seeded value-noise fBm cloud generated on the device (extension; the
  * reference ships no clouds, models/PUT_YOUR_MODELS_HERE.txt). */

@@ -23,6 +23,17 @@ def small_c1(cfg, S, w=40, h=24, size=48, sv_axis=40, **ext_kw):
     return sc
 
 
+def skip_reference(maj):
+    """Majorant for a non-empty brick, -min(16, max-norm distance to the nearest
+    non-empty brick) for an empty one (brute force)."""
+    occ = np.argwhere(maj > 0)
+    out = maj.astype(np.float32).copy()
+    for idx in np.argwhere(maj == 0):
+        d = 16 if len(occ) == 0 else min(16, int(np.abs(occ - idx).max(axis=1).min()))
+        out[tuple(idx)] = -float(d)
+    return out
+
+
 @pytest.mark.parametrize("storage,layout", [(0, 0), (1, 0), (1, 1), (0, 1)])
 def test_fbm_grid_bitexact(ctx, cfg, S, O, storage, layout):
     p = cfg.fbm((37, 29, 23), 1.0 / 37, seed=5, octaves=6)
@@ -40,6 +51,7 @@ def test_fbm_grid_bitexact(ctx, cfg, S, O, storage, layout):
         ctx.set_render_ext(cfg.render_ext(majorant_brick=b))
         maj, _ = O.brick_majorants(ref, b)
         assert_bitexact(ctx.majorants(), maj, f"brick majorants, {b}^3 bricks")
+        assert_bitexact(ctx.skip_grid(), skip_reference(maj), f"skip grid, {b}^3 bricks")
     c = ctx.cloud()
     assert c.maxExtinction == max(float(ref.max()), 0.01)
     assert tuple(c.voxelCount) == (37, 29, 23, 0)
