@@ -35,6 +35,10 @@ using namespace crd;
 namespace {
 
 constexpr uint32_t COLLISION_CEILING = 1u << 20;  // guards a non-terminating walk only
+#ifndef PT_DDA_STEPS_PER_PASS
+#define PT_DDA_STEPS_PER_PASS 8
+#endif
+constexpr uint32_t DDA_STEPS_PER_PASS = PT_DDA_STEPS_PER_PASS;  // brick-walk steps per tracking call
 
 enum : uint32_t { ST_IDLE = 0, ST_ACTIVE = 1, ST_DEAD = 2 };
 enum : uint32_t { MODE_PRIMARY = 0, MODE_SHADOW = 1 };
@@ -174,7 +178,11 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
         // that cube no majorant is fetched.  The arithmetic of the walk is
         // unchanged (an empty brick has majorant 0 either way).
         int32_t ex = 0, ey = 0, ez = 0, er = -1;
-        for (;;) {
+        // At most DDA_STEPS_PER_PASS bricks per call: a lane crossing a long
+        // stretch of thin or empty bricks resumes in the next pass instead of
+        // holding its wave's other lanes idle (the walk's state is in Lane).
+        for (uint32_t it = 0;; it++) {
+            if (it == DDA_STEPS_PER_PASS) return EV_NONE;
             float sb;
             if (abs(L.cell[0] - ex) <= er && abs(L.cell[1] - ey) <= er && abs(L.cell[2] - ez) <= er) {
                 maj = 0.0f;
@@ -419,8 +427,14 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
 
         // ---- divergent tail: events that need no lookup are resolved here,
         // possibly several in a row (e.g. a camera ray that misses the box).
+        // Each pass resolves one event per lane; the work several events end
+        // with (sampling a new direction, starting a tracking segment) is done
+        // once per pass for every lane that needs it, so lanes handling
+        // different events share those instructions.
         bool needSample = (L.mode == 0xFFu);
         while (ev != EV_NONE || needSample) {
+            bool doScatter = false;  // sample a new direction along which to continue
+            uint32_t seg = 0;        // start a segment: 1 along L.dir, 2 toward the sun
             if (needSample) {
                 needSample = false;
                 if (L.f >= (KMODE == K_CONTINUE ? a.spp : a.nFrames)) {
@@ -477,10 +491,8 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                     continue;
                 }
                 L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
-                if (!begin_segment<TRACK>(a, L, L.dir)) ev = EV_EXIT;
-                else break;
-            }
-            if (ev == EV_EXIT && L.mode == MODE_PRIMARY) {
+                seg = 1;
+            } else if (ev == EV_EXIT && L.mode == MODE_PRIMARY) {
                 // escape: PathTracer.comp:399-403
                 add_scaled(L.res, background(a.ext, L.dir), L.thr);
                 finish_sample<KMODE>(a, L);
@@ -488,17 +500,13 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 ev = EV_NONE;
                 needSample = true;
                 continue;
-            }
-            if (ev == EV_EXIT && L.mode == MODE_SHADOW) {
-                // shadow ray done: add NEE and scatter
+            } else if (ev == EV_EXIT && L.mode == MODE_SHADOW) {
+                // shadow ray done: add NEE, then scatter
                 add_scalar(L.res, ((L.ipdf * L.T) * L.w));
-                L.dir = scatter(a.params, L.dir, L.rng);
                 L.thr = L.w;
                 L.mode = MODE_PRIMARY;
-                ev = begin_segment<TRACK>(a, L, L.dir) ? EV_NONE : EV_EXIT;
-                continue;
-            }
-            if (ev == EV_COLLIDE) {
+                doScatter = true;
+            } else {  // EV_COLLIDE
                 if (KMODE == K_RECORD && L.bounces == 0) {
                     // first real collision: record it and end the frame here
                     float4* r = reinterpret_cast<float4*>(record_of(a, L));
@@ -521,27 +529,31 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 const float pdf = phase_pdf(a.params, neg(L.dir), ldir);
                 const float w = L.thr * albedo;
                 if (NEE == CR_NEE_RATIO_TRACKING) {
+                    // a shadow ray that misses the box keeps T = 1 and draws nothing
                     L.ipdf = (1.0f * I) * pdf;
                     L.w = w;
                     L.T = 1.0f;
                     L.mode = MODE_SHADOW;
-                    // a shadow ray that misses the box keeps T = 1 and draws nothing
-                    ev = begin_segment<TRACK>(a, L, sdir) ? EV_NONE : EV_EXIT;
-                    continue;
-                }
-                float Tv;
-                if (NEE == CR_NEE_SHADOW_VOLUME) {
-                    Tv = sample_sv(a, L.pos);
-                    c.svLookups++;
+                    seg = 2;
                 } else {
-                    Tv = 0.0f;
+                    float Tv;
+                    if (NEE == CR_NEE_SHADOW_VOLUME) {
+                        Tv = sample_sv(a, L.pos);
+                        c.svLookups++;
+                    } else {
+                        Tv = 0.0f;
+                    }
+                    add_scalar(L.res, (((1.0f * I) * pdf) * Tv) * w);
+                    L.thr = w;
+                    doScatter = true;
                 }
-                add_scalar(L.res, (((1.0f * I) * pdf) * Tv) * w);
-                L.dir = scatter(a.params, L.dir, L.rng);
-                L.thr = w;
-                ev = begin_segment<TRACK>(a, L, L.dir) ? EV_NONE : EV_EXIT;
-                continue;
             }
+            if (doScatter) {
+                L.dir = scatter(a.params, L.dir, L.rng);
+                seg = 1;
+            }
+            // seg != 0 here: every branch that falls through starts a segment
+            ev = begin_segment<TRACK>(a, L, seg == 2 ? sdir : L.dir) ? EV_NONE : EV_EXIT;
         }
     }
 
