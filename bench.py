@@ -247,6 +247,8 @@ def bench_c5(args, pkg, cfg):
     for i in range(0, args.c5_ref_spp, 256):
         ctx.pt_render(ref_frames[i:i + 256])
         ctx.synchronize()
+        if (i // 256) % 16 == 15:
+            log(f"c5: reference {i + 256} spp, {time.perf_counter() - t0:.1f}s")
     ref = ctx.download_image(H, W)
     t_ref = time.perf_counter() - t0
     log(f"c5: reference {args.c5_ref_spp} spp in {t_ref:.1f}s")
