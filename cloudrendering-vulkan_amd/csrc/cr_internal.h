@@ -18,7 +18,7 @@ struct DeviceGrid {
     int32_t nbx = 0, nby = 0, nbz = 0;
     size_t bytes = 0;
     float* majorants = nullptr; // ceil(N/mb)^3 brick maxima (with apron)
-    float* skip = nullptr;      // the same, empty bricks as -(Chebyshev distance to a non-empty one, <= 16)
+    float* skip = nullptr;      // (majorant * ds) / bs; empty bricks -(Chebyshev distance to a non-empty one, <= 16)
     int32_t mx = 0, my = 0, mz = 0;
     int32_t mb = 16;            // voxels per side of a majorant brick
 };
@@ -106,9 +106,12 @@ hipError_t launch_grid_to_f32(const void* src, int dtype, int layout, float* dst
 hipError_t launch_majorants(const void* grid, int dtype, int layout, int32_t nx, int32_t ny, int32_t nz, int32_t nbx,
                             int32_t nby, int32_t mb, float* out, int32_t mx, int32_t my, int32_t mz, hipStream_t s);
 hipError_t launch_math_probe(int op, const float* in, uint32_t n, float* out, hipStream_t s);
-// Skip grid from the majorants (tmp: 2 * n bytes of scratch).
-hipError_t launch_skip_grid(const float* majorants, int32_t mx, int32_t my, int32_t mz, float* skip, uint8_t* tmp,
-                            hipStream_t s);
+// Skip grid from the majorants (tmp: 2 * n bytes of scratch); ds / bs are the
+// cloud's densityScaling and baseScaling.  launch_skip_scale redoes only the
+// non-empty bricks' values after those change.
+hipError_t launch_skip_grid(const float* majorants, int32_t mx, int32_t my, int32_t mz, float ds, float bs, float* skip,
+                            uint8_t* tmp, hipStream_t s);
+hipError_t launch_skip_scale(const float* majorants, uint32_t n, float ds, float bs, float* skip, hipStream_t s);
 hipError_t launch_grid_max(const float* majorants, uint32_t n, unsigned int* out_bits, hipStream_t s);
 
 } // namespace cr

@@ -240,7 +240,8 @@ cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb)
                                  ctx->stream), "majorant kernel");
     uint8_t* tmp = nullptr;
     CR_HIP(ctx, hipMallocAsync((void**)&tmp, 2 * n, ctx->stream), "skip scratch");
-    hipError_t e = launch_skip_grid(g.majorants, mx, my, mz, g.skip, tmp, ctx->stream);
+    hipError_t e = launch_skip_grid(g.majorants, mx, my, mz, ctx->cloud.densityScaling, ctx->cloud.baseScaling, g.skip, tmp,
+                                    ctx->stream);
     hipError_t e2 = hipFreeAsync(tmp, ctx->stream);
     CR_HIP(ctx, e, "skip grid");
     CR_HIP(ctx, e2, "free skip scratch");
@@ -485,10 +486,19 @@ cr_status cr_set_cloud(cr_context* ctx, const cr_cloud_properties* c)
 {
     CR_CHECK_CTX(ctx);
     if (!c) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null cloud properties");
-    if (!(c->baseScaling != 0.0f) || !(c->maxExtinction > 0.0f))
-        return fail(ctx, CR_ERR_INVALID_ARGUMENT, "baseScaling must be nonzero and maxExtinction positive");
+    if (!(c->baseScaling > 0.0f) || !(c->maxExtinction > 0.0f) || !(c->densityScaling >= 0.0f) ||
+        !(c->densityScaling < __builtin_inff()))
+        return fail(ctx, CR_ERR_INVALID_ARGUMENT,
+                    "baseScaling and maxExtinction must be positive, densityScaling finite and non-negative");
+    const bool rescale = ctx->hasGrid && (c->densityScaling != ctx->cloud.densityScaling ||
+                                          c->baseScaling != ctx->cloud.baseScaling);
     ctx->cloud = *c;
     ctx->svValid = false;
+    if (rescale) {
+        const DeviceGrid& g = ctx->grid;
+        CR_HIP(ctx, launch_skip_scale(g.majorants, (uint32_t)((size_t)g.mx * g.my * g.mz), c->densityScaling, c->baseScaling,
+                                      g.skip, ctx->stream), "skip grid scale");
+    }
     return CR_OK;
 }
 

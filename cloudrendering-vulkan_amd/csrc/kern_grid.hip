@@ -179,13 +179,24 @@ __global__ void __launch_bounds__(256) majorant_kernel(const T* g, int32_t nx, i
 // Skip grid: occupancy, then 15 rounds of a 3x3x3 dilation (one axis per
 // pass); a brick first covered in round d is d bricks (Chebyshev) from the
 // nearest non-empty brick.  Bricks still uncovered keep -16.
-__global__ void __launch_bounds__(256) skip_init_kernel(const float* maj, uint32_t n, uint8_t* occ, float* skip)
+// A non-empty brick stores its extinction majorant (m * densityScaling) /
+// baseScaling, the value the brick walk needs.
+__global__ void __launch_bounds__(256) skip_init_kernel(const float* maj, uint32_t n, uint8_t* occ, float* skip, float ds,
+                                                        float bs)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float m = maj[i];
     occ[i] = m > 0.0f ? 1 : 0;
-    skip[i] = m > 0.0f ? m : -16.0f;
+    skip[i] = m > 0.0f ? fdiv(m * ds, bs) : -16.0f;
+}
+
+__global__ void __launch_bounds__(256) skip_scale_kernel(const float* maj, uint32_t n, float* skip, float ds, float bs)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float m = maj[i];
+    if (m > 0.0f) skip[i] = fdiv(m * ds, bs);
 }
 
 __global__ void __launch_bounds__(256) dilate_kernel(const uint8_t* in, uint8_t* out, int32_t mx, int32_t my, int32_t mz,
@@ -298,14 +309,14 @@ hipError_t launch_math_probe(int op, const float* in, uint32_t n, float* out, hi
     return hipGetLastError();
 }
 
-hipError_t launch_skip_grid(const float* majorants, int32_t mx, int32_t my, int32_t mz, float* skip, uint8_t* tmp,
-                            hipStream_t s)
+hipError_t launch_skip_grid(const float* majorants, int32_t mx, int32_t my, int32_t mz, float ds, float bs, float* skip,
+                            uint8_t* tmp, hipStream_t s)
 {
     const uint32_t n = (uint32_t)mx * (uint32_t)my * (uint32_t)mz;
     const dim3 g((n + 255) / 256);
     uint8_t* a = tmp;
     uint8_t* b = tmp + n;
-    hipLaunchKernelGGL(skip_init_kernel, g, dim3(256), 0, s, majorants, n, a, skip);
+    hipLaunchKernelGGL(skip_init_kernel, g, dim3(256), 0, s, majorants, n, a, skip, ds, bs);
     for (int d = 1; d <= 15; d++) {
         hipLaunchKernelGGL(dilate_kernel, g, dim3(256), 0, s, a, b, mx, my, mz, 0, (float*)nullptr, 0.0f);
         hipLaunchKernelGGL(dilate_kernel, g, dim3(256), 0, s, b, a, mx, my, mz, 1, (float*)nullptr, 0.0f);
@@ -314,6 +325,12 @@ hipError_t launch_skip_grid(const float* majorants, int32_t mx, int32_t my, int3
         a = b;
         b = t;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_skip_scale(const float* majorants, uint32_t n, float ds, float bs, float* skip, hipStream_t s)
+{
+    hipLaunchKernelGGL(skip_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, s, majorants, n, skip, ds, bs);
     return hipGetLastError();
 }
 

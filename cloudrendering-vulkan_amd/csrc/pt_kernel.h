@@ -87,6 +87,15 @@ __device__ __forceinline__ float sample_sv(const cr::PTArgs& a, f3 p)
     return trilinear<float, LAYOUT_DENSE>(g, sv_norm(a.svp, p));
 }
 
+// bw / |direction| per axis for the lane's current segment (computed once per
+// segment by dda_init with the same division every brick step would make),
+// kept in LDS to spare registers.
+__device__ __forceinline__ float* tdelta_lds()
+{
+    __shared__ float td[256 * 3];
+    return td;
+}
+
 __device__ __forceinline__ void dda_init(const cr::PTArgs& a, f3 pos, f3 dir, Lane& L)
 {
     const float p[3] = {pos.x, pos.y, pos.z};
@@ -98,13 +107,18 @@ __device__ __forceinline__ void dda_init(const cr::PTArgs& a, f3 pos, f3 dir, La
         int32_t c = (int32_t)floorf(fminf(fmaxf(rel, 0.0f), (float)mc[k]));
         c = clampi(c, 0, mc[k] - 1);
         L.cell[k] = c;
+        float td;
         if (v[k] > 0.0f) {
             L.tNext[k] = fmaxf(fdiv((B0(a)[k] + (float)(c + 1) * a.bw[k]) - p[k], v[k]), 0.0f);
+            td = fdiv(a.bw[k], v[k]);
         } else if (v[k] < 0.0f) {
             L.tNext[k] = fmaxf(fdiv((B0(a)[k] + (float)c * a.bw[k]) - p[k], v[k]), 0.0f);
+            td = fdiv(a.bw[k], -v[k]);
         } else {
             L.tNext[k] = __builtin_inff();
+            td = __builtin_inff();
         }
+        tdelta_lds()[threadIdx.x * 3 + k] = td;
     }
 }
 
@@ -113,11 +127,7 @@ __device__ __forceinline__ void dda_step_axis(const cr::PTArgs& a, float v, int3
 {
     int32_t c = L.cell[K] + (v > 0.0f ? 1 : (v < 0.0f ? -1 : 0));
     L.cell[K] = c;
-    float d;
-    if (v > 0.0f) d = fdiv(a.bw[K], v);
-    else if (v < 0.0f) d = fdiv(a.bw[K], -v);
-    else d = __builtin_inff();
-    L.tNext[K] = L.tNext[K] + d;
+    L.tNext[K] = L.tNext[K] + tdelta_lds()[threadIdx.x * 3 + K];
 }
 
 // Advance to the neighbouring brick along the axis of the nearest boundary
@@ -136,12 +146,16 @@ __device__ __forceinline__ bool dda_advance(const cr::PTArgs& a, f3 dir, Lane& L
     return L.cell[2] >= 0 && L.cell[2] < a.mz;
 }
 
-// Brick majorant, or -d for an empty brick d bricks from the nearest
-// non-empty one (the skip grid).
-__device__ __forceinline__ float brick_majorant(const cr::PTArgs& a, const Lane& L)
+__device__ __forceinline__ uint32_t brick_index(const cr::PTArgs& a, const Lane& L)
 {
-    return a.skip[(uint32_t)L.cell[0] + (uint32_t)a.mx * ((uint32_t)L.cell[1] + (uint32_t)a.my * (uint32_t)L.cell[2])];
+    return (uint32_t)L.cell[0] + (uint32_t)a.mx * ((uint32_t)L.cell[1] + (uint32_t)a.my * (uint32_t)L.cell[2]);
 }
+
+__device__ __forceinline__ float brick_majorant(const cr::PTArgs& a, const Lane& L) { return a.majorants[brick_index(a, L)]; }
+
+// The skip grid: (majorant * densityScaling) / baseScaling of a non-empty
+// brick, -d for an empty brick d bricks from the nearest non-empty one.
+__device__ __forceinline__ float brick_skip(const cr::PTArgs& a, const Lane& L) { return a.skip[brick_index(a, L)]; }
 
 // Start a tracking segment from L.pos along `dir` (primary or shadow).
 // Returns false when the segment misses the box (no RNG draw is made then,
@@ -183,21 +197,19 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
         // holding its wave's other lanes idle (the walk's state is in Lane).
         for (uint32_t it = 0;; it++) {
             if (it == DDA_STEPS_PER_PASS) return EV_NONE;
+            // sb = (majorant * densityScaling) / baseScaling, precomputed per
+            // brick in the skip grid (the same IEEE division as here)
             float sb;
             if (abs(L.cell[0] - ex) <= er && abs(L.cell[1] - ey) <= er && abs(L.cell[2] - ez) <= er) {
-                maj = 0.0f;
                 sb = 0.0f;
             } else {
-                maj = brick_majorant(a, L);
-                if (maj < 0.0f) {
+                sb = brick_skip(a, L);
+                if (sb < 0.0f) {
                     ex = L.cell[0];
                     ey = L.cell[1];
                     ez = L.cell[2];
-                    er = (int32_t)(-maj) - 1;
-                    maj = 0.0f;
+                    er = (int32_t)(-sb) - 1;
                     sb = 0.0f;
-                } else {
-                    sb = fdiv(maj * a.cloud.densityScaling, a.cloud.baseScaling);
                 }
             }
             float tc = fminf(fminf(fminf(L.tNext[0], L.tNext[1]), L.tNext[2]), L.dist);
@@ -212,6 +224,7 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
             if (!(tc < L.dist)) return EV_EXIT;
             if (!dda_advance(a, dir, L)) return EV_EXIT;
         }
+        maj = brick_majorant(a, L);
         f3 cur = add(L.pos, mul(dir, L.t));
         float e = sample_cloud<T, LAYOUT>(a, cur);
         c.lookups++;

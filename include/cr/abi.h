@@ -131,9 +131,10 @@ CR_API cr_status cr_grid_download(cr_context* ctx, float* out);
 /* Brick-majorant grid (extension): ceil(N/b) per axis for b-voxel bricks, x
  * fastest, max density over each b^3 brick plus a one-voxel apron. */
 CR_API cr_status cr_grid_majorants(cr_context* ctx, uint32_t dims_out[3], float* out /* may be NULL */);
-/* The empty-space skip grid the brick walk reads (extension): the majorant of
- * a non-empty brick, and -d for an empty brick whose nearest non-empty brick
- * is d bricks away in the max norm (d <= 16; 16 = at least 16). */
+/* The empty-space skip grid the brick walk reads (extension): for a non-empty
+ * brick its extinction majorant (majorant * densityScaling) / baseScaling, for
+ * an empty brick -d, d being the max-norm distance in bricks to the nearest
+ * non-empty brick (d <= 16; 16 = at least 16). */
 CR_API cr_status cr_grid_skip(cr_context* ctx, float* out);
 
 /* Seeded value-noise fBm cloud generated on the device (extension; the
@@ -158,6 +159,7 @@ CR_API cr_status cr_grid_generate_fbm(cr_context* ctx, const cr_fbm_params* p, i
  * RenderTechnique::QueueUpdate{CameraProperties,CloudData,Parameters,
  * ShadowVolume} + UpdateDescriptorSets (RenderTechnique.h:41-46). */
 CR_API cr_status cr_set_camera(cr_context* ctx, const cr_camera_properties* cam);
+/* baseScaling and maxExtinction must be > 0, densityScaling finite and >= 0. */
 CR_API cr_status cr_set_cloud(cr_context* ctx, const cr_cloud_properties* cloud);
 CR_API cr_status cr_get_cloud(cr_context* ctx, cr_cloud_properties* cloud);
 CR_API cr_status cr_set_parameters(cr_context* ctx, const cr_parameters* params);

@@ -23,11 +23,11 @@ def small_c1(cfg, S, w=40, h=24, size=48, sv_axis=40, **ext_kw):
     return sc
 
 
-def skip_reference(maj):
-    """Majorant for a non-empty brick, -min(16, max-norm distance to the nearest
-    non-empty brick) for an empty one (brute force)."""
+def skip_reference(maj, ds, bs):
+    """(majorant * ds) / bs in float32 for a non-empty brick, -min(16, max-norm
+    distance to the nearest non-empty brick) for an empty one (brute force)."""
     occ = np.argwhere(maj > 0)
-    out = maj.astype(np.float32).copy()
+    out = ((maj.astype(np.float32) * np.float32(ds)) / np.float32(bs)).astype(np.float32)
     for idx in np.argwhere(maj == 0):
         d = 16 if len(occ) == 0 else min(16, int(np.abs(occ - idx).max(axis=1).min()))
         out[tuple(idx)] = -float(d)
@@ -51,7 +51,13 @@ def test_fbm_grid_bitexact(ctx, cfg, S, O, storage, layout):
         ctx.set_render_ext(cfg.render_ext(majorant_brick=b))
         maj, _ = O.brick_majorants(ref, b)
         assert_bitexact(ctx.majorants(), maj, f"brick majorants, {b}^3 bricks")
-        assert_bitexact(ctx.skip_grid(), skip_reference(maj), f"skip grid, {b}^3 bricks")
+        cl = ctx.cloud()
+        assert_bitexact(ctx.skip_grid(), skip_reference(maj, cl.densityScaling, cl.baseScaling), f"skip grid, {b}^3 bricks")
+    # a new density scale rescales the non-empty bricks only
+    cl = ctx.cloud()
+    cl.densityScaling = 37.5
+    ctx.set_cloud(cl)
+    assert_bitexact(ctx.skip_grid(), skip_reference(maj, 37.5, cl.baseScaling), "skip grid after cr_set_cloud")
     c = ctx.cloud()
     assert c.maxExtinction == max(float(ref.max()), 0.01)
     assert tuple(c.voxelCount) == (37, 29, 23, 0)
