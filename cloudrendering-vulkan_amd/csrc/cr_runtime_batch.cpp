@@ -117,15 +117,16 @@ std::vector<uint32_t> table_order(const Batch* b)
 uint32_t tile_log2(uint32_t tileWidth) { return tileWidth == 32 ? 5u : (tileWidth == 16 ? 4u : 3u); }
 
 // Job table for `which` jobs (table order); images at `images[k]` if given
-// (trial renders), frames limited to `max_frames` (0 = all).
+// (trial renders), each job's frame count divided by `frame_div` (at least 1
+// frame; 0 = all frames).
 std::vector<PTArgs> build_table(const Batch* b, const std::vector<uint32_t>& which, uint32_t tileWidth,
-                                const std::vector<float*>* images, uint32_t max_frames, uint32_t* total)
+                                const std::vector<float*>* images, uint32_t frame_div, uint32_t* total)
 {
     std::vector<PTArgs> t(which.size());
     uint32_t base = 0;
     for (size_t k = 0; k < which.size(); k++) {
         const BatchJob& j = b->jobs[which[k]];
-        const uint32_t nf = max_frames ? std::min(max_frames, j.nFrames) : j.nFrames;
+        const uint32_t nf = frame_div ? std::max(1u, j.nFrames / frame_div) : j.nFrames;
         fill_job_args(t[k], j.grid, j.sv, j.svDims, j.cam, j.cloud, j.params, j.svp, j.ext, j.dFrames, nf,
                       images ? (*images)[k] : j.image, j.W, j.H, tile_log2(tileWidth));
         t[k].base = base;
@@ -154,8 +155,11 @@ cr_status upload_table(cr_context* ctx, PTArgs* dst, const std::vector<PTArgs>& 
     return CR_OK;
 }
 
-// One trial: `which` jobs, one frame each, into the scratch images; returns the
-// best of `reps` timed launches (after one untimed one).
+// One trial: `which` jobs, each with 1/TRIAL_FRAME_DIV of its frames (so the
+// trial has the real mix of long and short pixels), into the scratch images;
+// returns the best of `reps` timed launches (after one untimed one).
+constexpr uint32_t TRIAL_FRAME_DIV = 32;
+
 cr_status trial(cr_context* ctx, Batch* b, const std::vector<uint32_t>& which, const cr_batch_tuning& t, int reps,
                 float* best_ms)
 {
@@ -166,7 +170,7 @@ cr_status trial(cr_context* ctx, Batch* b, const std::vector<uint32_t>& which, c
         off += (size_t)b->jobs[which[k]].W * b->jobs[which[k]].H * 4;
     }
     uint32_t total = 0;
-    std::vector<PTArgs> table = build_table(b, which, t.tileWidth, &images, 1, &total);
+    std::vector<PTArgs> table = build_table(b, which, t.tileWidth, &images, TRIAL_FRAME_DIV, &total);
     cr_status st = upload_table(ctx, b->dTuneTable, table);
     if (st != CR_OK) return st;
     BatchArgs a;
@@ -548,7 +552,7 @@ cr_status cr_batch_autotune(cr_context* ctx, int tune_bricks, cr_batch_tuning* o
                 if (st != CR_OK) return st;
                 j.ext.majorantBrick = (uint32_t)mb;
                 float ms = 0.0f;
-                st = trial(ctx, b, std::vector<uint32_t>{k}, def, 2, &ms);
+                st = trial(ctx, b, std::vector<uint32_t>{k}, def, 1, &ms);
                 if (st != CR_OK) return st;
                 if (ms < bestMs) {
                     bestMs = ms;
@@ -565,7 +569,7 @@ cr_status cr_batch_autotune(cr_context* ctx, int tune_bricks, cr_batch_tuning* o
 
     // (2) launch shape over the whole batch
     const std::vector<uint32_t> order = table_order(b);
-    st = trial(ctx, b, order, def, 2, &rep.defaultMs);
+    st = trial(ctx, b, order, def, 1, &rep.defaultMs);
     if (st != CR_OK) return st;
     cr_batch_tuning best = def;
     float bestMs = rep.defaultMs;
@@ -574,7 +578,8 @@ cr_status cr_batch_autotune(cr_context* ctx, int tune_bricks, cr_batch_tuning* o
     for (uint32_t th : threads) {
         const int occ = pt_batch_blocks_per_cu(b->dtype, b->layout, b->tracking, b->nee, (int)th,
                                                (uint32_t)b->jobs.size());
-        for (int per = 1; per <= occ; per++) {
+        // below half the occupancy limit a persistent grid cannot hide its latency
+        for (int per = std::max(1, occ / 2); per <= occ; per++) {
             for (uint32_t tw : tiles) {
                 cr_batch_tuning t;
                 t.blockThreads = th;
@@ -582,7 +587,7 @@ cr_status cr_batch_autotune(cr_context* ctx, int tune_bricks, cr_batch_tuning* o
                 t.tileWidth = tw;
                 t._reserved = 0;
                 float ms = 0.0f;
-                st = trial(ctx, b, order, t, 2, &ms);
+                st = trial(ctx, b, order, t, 1, &ms);
                 if (st != CR_OK) return st;
                 rep.candidates++;
                 if (ms < bestMs) {

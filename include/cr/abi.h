@@ -234,7 +234,7 @@ typedef struct cr_batch_tuning {
     uint32_t _reserved;
 } cr_batch_tuning;
 typedef struct cr_batch_tune_report {
-    float defaultMs;       /* one frame of every job with the default tuning */
+    float defaultMs;       /* the trial render (1/32 of every job's frames) with the default tuning */
     float bestMs;          /* the same with the chosen tuning */
     uint32_t candidates;   /* launch shapes timed */
     uint32_t jobBrick[CR_BATCH_MAX_JOBS]; /* chosen majorant brick per job (insertion order) */
@@ -259,8 +259,9 @@ CR_API cr_status cr_batch_render(cr_context* ctx);
 CR_API cr_status cr_batch_download(cr_context* ctx, uint32_t job, float* rgba, uint32_t* width, uint32_t* height);
 CR_API cr_status cr_batch_stats_get(cr_context* ctx, cr_pt_stats* out);
 CR_API cr_status cr_batch_stats_reset(cr_context* ctx);
-/* Autotuner: times one frame of every job (into scratch images; the jobs'
- * images are untouched) for each launch shape (blockThreads x blocksPerCU x
+/* Autotuner: times a trial render of 1/32 of every job's frames (at least
+ * one; into scratch images, the jobs' images are untouched) for each launch
+ * shape (blockThreads x blocksPerCU from half the occupancy limit up x
  * tileWidth) and keeps the fastest.  With tune_bricks, first times each job
  * alone with majorant bricks of 8, 16 and 32 voxels and keeps the fastest
  * (this changes the job's extension block: the job then equals cr_pt_render
