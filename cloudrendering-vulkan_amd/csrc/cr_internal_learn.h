@@ -14,6 +14,7 @@ constexpr int LEARN_LEVELS = 8;
 struct FeatureArgs {
     const float* points;    // [n][8]: x, y, z, omega_x, omega_y, omega_z, -, -
     uint64_t n;
+    const uint32_t* ndev;   // when set, the point count is *ndev (<= n, n sizes the launch)
     const float* mips[LEARN_LEVELS];
     int32_t dims[LEARN_LEVELS][3];
     cr_cloud_properties cloud;
@@ -38,6 +39,7 @@ hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
 hipError_t launch_compose(const ComposeArgs& a, hipStream_t s);
 hipError_t launch_count(const float* rec, uint64_t n, uint32_t* counts, hipStream_t s);
 hipError_t launch_compact(const float* rec, uint64_t n, const uint32_t* offsets, float* pts, uint32_t* slot, hipStream_t s);
+hipError_t launch_scan_counts(uint32_t* counts, uint32_t nb, uint32_t* total, hipStream_t s);
 hipError_t launch_select(const float* src, uint32_t stride, uint32_t count, float* dst, hipStream_t s);
 hipError_t launch_append(const uint16_t* X, const float4* T4, uint32_t stride, uint32_t count, uint16_t* dsX, float* dsT,
                          uint64_t offset, hipStream_t s);

@@ -17,6 +17,7 @@ constexpr uint32_t MLP_N[MLP_LAYERS] = {128, 128, 128, 128, 16};
 struct MlpArgs {
     const uint16_t* X;       // [B][224] bf16 features
     int64_t B;
+    const uint32_t* Bdev;    // forward only: when set, the row count is *Bdev (<= B, B sizes the launch)
     uint16_t* WT[MLP_LAYERS];  // bf16 [N][K] (forward B fragments)
     uint16_t* W[MLP_LAYERS];   // bf16 [K][N] (backward B fragments)
     const float* bias[MLP_LAYERS];

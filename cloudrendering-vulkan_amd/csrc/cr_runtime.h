@@ -92,7 +92,8 @@ void mlp_free(cr_context* ctx);
 void learn_free(cr_context* ctx);
 void batch_free(cr_context* ctx);
 cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr);
-cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y);
+// Bdev: optional device-resident row count (<= B; B sizes the launch).
+cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y, const uint32_t* Bdev = nullptr);
 cr_status mlp_last_loss(cr_context* ctx, float* loss);
 cr_status fill_pt_args(cr_context* ctx, PTArgs& a, uint32_t n_frames);
 } // namespace cr

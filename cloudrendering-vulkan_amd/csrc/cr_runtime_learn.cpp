@@ -29,6 +29,8 @@ struct Learn {
     size_t slotCap = 0;
     uint32_t* counts = nullptr;
     size_t countsCap = 0;
+    uint32_t* mdev = nullptr;   // collision count of the last device-side compaction
+    size_t mdevCap = 0;
     float* Y = nullptr;
     size_t YCap = 0;
     float* targets = nullptr;
@@ -52,7 +54,8 @@ void learn_free(cr_context* ctx)
     if (!L) return;
     for (int k = 0; k < LEARN_LEVELS; k++)
         if (L->mips[k]) hipFree(L->mips[k]);
-    void* bufs[] = {L->points, L->X, L->records, L->slot, L->counts, L->Y, L->targets, L->dsX, L->dsT, L->bX, L->bT};
+    void* bufs[] = {L->points, L->X, L->records, L->slot, L->counts, L->mdev, L->Y, L->targets, L->dsX, L->dsT, L->bX,
+                    L->bT};
     for (void* b : bufs)
         if (b) hipFree(b);
     delete L;
@@ -88,8 +91,10 @@ cr_status learn_mips(cr_context* ctx)
     return CR_OK;
 }
 
-// Features of device-resident points into device-resident X.
-cr_status learn_features_device(cr_context* ctx, const float* points, uint64_t n, uint16_t* X)
+// Features of device-resident points into device-resident X; ndev: optional
+// device-resident point count (<= n, n sizes the launch).
+cr_status learn_features_device(cr_context* ctx, const float* points, uint64_t n, uint16_t* X,
+                                const uint32_t* ndev = nullptr)
 {
     Learn* L = learn_get(ctx);
     if (!L->valid || L->builtVersion != ctx->gridVersion)
@@ -98,6 +103,7 @@ cr_status learn_features_device(cr_context* ctx, const float* points, uint64_t n
     memset(&a, 0, sizeof(a));
     a.points = points;
     a.n = n;
+    a.ndev = ndev;
     for (int k = 0; k < LEARN_LEVELS; k++) {
         a.mips[k] = L->mips[k];
         a.dims[k][0] = L->dims[k][0]; a.dims[k][1] = L->dims[k][1]; a.dims[k][2] = L->dims[k][2];
@@ -232,6 +238,22 @@ cr_status compact(cr_context* ctx, Learn* L, uint64_t n, uint32_t* m_out)
     CR_HIP(ctx, launch_compact(L->records, n, L->counts, L->points, L->slot, ctx->stream), "compact collisions");
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");  // c is read by the copy above
     *m_out = (uint32_t)total;
+    return CR_OK;
+}
+
+// The same compaction with the scan on the device: no host round trip; the
+// count stays in L->mdev and L->points / L->slot are sized for all n records.
+cr_status compact_device(cr_context* ctx, Learn* L, uint64_t n)
+{
+    const uint64_t nb = (n + 255) / 256;
+    if (nb > 0xFFFFFFFFull) return fail(ctx, CR_ERR_CAPACITY, "too many records in one pass");
+    CR_TRY(ensure_buffer(ctx, (void**)&L->counts, &L->countsCap, nb * 4, "counts"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->slot, &L->slotCap, n * 4, "slots"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->mdev, &L->mdevCap, 4, "collision count"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->points, &L->pointsCap, n * 32, "points"));
+    CR_HIP(ctx, launch_count(L->records, n, L->counts, ctx->stream), "count collisions");
+    CR_HIP(ctx, launch_scan_counts(L->counts, (uint32_t)nb, L->mdev, ctx->stream), "scan counts");
+    CR_HIP(ctx, launch_compact(L->records, n, L->counts, L->points, L->slot, ctx->stream), "compact collisions");
     return CR_OK;
 }
 
@@ -402,16 +424,16 @@ cr_status cr_learn_render(cr_context* ctx, const cr_push_constants* frames, uint
     Learn* L = learn_get(ctx);
     CR_TRY(check_render_state(ctx));
     CR_TRY(ensure_mips(ctx, L));
+    // record -> compaction -> features -> network -> composition, all on the
+    // stream: the collision count stays on the device and bounds the feature
+    // and network launches, which are sized for every record.
     uint64_t n = 0;
-    uint32_t m = 0;
     CR_TRY(record_pass(ctx, L, frames, n_frames, &n));
-    CR_TRY(compact(ctx, L, n, &m));
-    if (m > 0) {
-        CR_TRY(ensure_buffer(ctx, (void**)&L->X, &L->XCap, (size_t)m * 448, "features"));
-        CR_TRY(ensure_buffer(ctx, (void**)&L->Y, &L->YCap, (size_t)m * 16, "network outputs"));
-        CR_TRY(learn_features_device(ctx, L->points, m, L->X));
-        CR_TRY(mlp_forward_device(ctx, L->X, m, L->Y));
-    }
+    CR_TRY(compact_device(ctx, L, n));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->X, &L->XCap, (size_t)n * 448, "features"));
+    CR_TRY(ensure_buffer(ctx, (void**)&L->Y, &L->YCap, (size_t)n * 16, "network outputs"));
+    CR_TRY(learn_features_device(ctx, L->points, n, L->X, L->mdev));
+    CR_TRY(mlp_forward_device(ctx, L->X, n, L->Y, L->mdev));
     ComposeArgs a;
     a.image = reinterpret_cast<float4*>(ctx->image);
     a.pixels = (uint32_t)(ctx->W * ctx->H);

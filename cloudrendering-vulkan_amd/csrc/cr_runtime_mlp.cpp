@@ -166,13 +166,14 @@ void mlp_free(cr_context* ctx)
 }
 
 // Forward pass on device-resident features; Y receives [B][4] fp32.
-cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y)
+cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y, const uint32_t* Bdev)
 {
     Mlp* m;
     cr_status st = mlp_get(ctx, &m);
     if (st != CR_OK) return st;
     MlpArgs a = args_of(m, X, B);
     a.Y = Y;
+    a.Bdev = Bdev;
     hipError_t e = launch_mlp_forward(a, false, ctx->stream);
     return e == hipSuccess ? CR_OK : hip_fail(ctx, e, "mlp forward");
 }

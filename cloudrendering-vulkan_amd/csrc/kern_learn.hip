@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(256) mip_kernel(const float* src, int32_t sx, 
 __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
 {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.n) return;
+    if (i >= (a.ndev ? (uint64_t)*a.ndev : a.n)) return;
     const float* pt = a.points + i * 8;
     const f3 x = mk(pt[0], pt[1], pt[2]);
     const f3 w = mk(pt[3], pt[4], pt[5]);
@@ -161,6 +161,33 @@ __global__ void __launch_bounds__(256) compact_kernel(const float* rec, uint64_t
     dst[1] = src[1];
 }
 
+// Exclusive scan of nb block counts in place, in one workgroup of 1024
+// threads (each scans a contiguous run); *total = the sum.
+__global__ void __launch_bounds__(1024) scan_counts_kernel(uint32_t* counts, uint32_t nb, uint32_t* total)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023u) / 1024u;
+    const uint32_t b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; i++) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - s;  // exclusive prefix of this thread's run
+    for (uint32_t i = b0; i < b1; i++) {
+        const uint32_t c = counts[i];
+        counts[i] = run;
+        run += c;
+    }
+    if (t == 1023u) *total = part[1023];
+}
+
 // Rows k*stride (k < count) of 8-float records copied to dst[k].
 __global__ void __launch_bounds__(256) select_kernel(const float* src, uint32_t stride, uint32_t count, float* dst)
 {
@@ -220,6 +247,12 @@ hipError_t launch_compact(const float* rec, uint64_t n, const uint32_t* offsets,
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(compact_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rec, n, offsets, pts, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_counts(uint32_t* counts, uint32_t nb, uint32_t* total, hipStream_t s)
+{
+    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, counts, nb, total);
     return hipGetLastError();
 }
 

@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(256) mlp_fwd_kernel(cr::MlpArgs a)
     __shared__ uint16_t lds[4 * 32 * ROW_STRIDE];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * 32;
+    if (a.Bdev) a.B = (int64_t)*a.Bdev;
     if (row0 >= a.B) return;
     uint16_t* act = lds + wave * 32 * ROW_STRIDE;
     // X rows -> LDS (28 x 16-B chunks per 224-wide row)
