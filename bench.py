@@ -325,13 +325,13 @@ def bench_c5(args, pkg, cfg):
     log(f"c5: reference {args.c5_ref_spp} spp in {t_ref:.1f}s")
 
     def progressive(render, max_spp, chunk):
-        """(spp, seconds, relMSE) after each step of a 1, 2, 3, 4, 6, 8, 11, ... schedule;
+        """(spp, seconds, relMSE) after each step of a 1, 2, 3, 4, 5, 6, 7, 9, ... (x1.2) schedule;
         stops at relMSE <= 1e-3, or when the error has stopped falling."""
         ctx.clear_image()
         ctx.synchronize()
         out, done, secs = [], 0, 0.0
         frames = cfg.frames(max_spp)
-        sched = sorted({int(round(2 ** (i / 2.0))) for i in range(0, 64)})
+        sched = sorted({max(1, int(round(1.2 ** i))) for i in range(0, 64)})
         for target in sched:
             if target > max_spp:
                 break
@@ -345,7 +345,7 @@ def bench_c5(args, pkg, cfg):
             out.append((done, secs, e))
             if e <= 1e-3:
                 break
-            if len(out) >= 5 and e > 0.97 * out[-3][2]:
+            if len(out) >= 8 and e > 0.97 * out[-5][2]:
                 break  # a floor: the estimator's bias, not its noise
         return out
 
@@ -355,11 +355,32 @@ def bench_c5(args, pkg, cfg):
     def first_below(curve):
         for spp, secs, e in curve:
             if e <= 1e-3:
-                return spp, secs
-        return None, None
+                return spp
+        return None
 
-    p_spp, p_s = first_below(plain)
-    l_spp, l_s = first_below(learned)
+    def timed(render, spp, chunk):
+        """Wall time of one render of exactly spp frames (the quality the
+        progressive pass found sufficient), as a renderer that knows its
+        target would run it; buffers are warm."""
+        if spp is None:
+            return None, None
+        frames = cfg.frames(spp)
+        best = None
+        for _ in range(2):
+            ctx.clear_image()
+            ctx.synchronize()
+            t = time.perf_counter()
+            for i in range(0, spp, chunk):
+                render(frames[i:i + chunk])
+            ctx.synchronize()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        return best, rel_mse(ctx.download_image(H, W), ref)
+
+    p_spp, l_spp = first_below(plain), first_below(learned)
+    p_s, p_e = timed(ctx.pt_render, p_spp, 256)
+    l_s, l_e = timed(ctx.learn_render, l_spp, 8)
+    log(f"c5: plain {p_spp} spp in {p_s}s (relMSE {p_e}), learned {l_spp} spp in {l_s}s (relMSE {l_e})")
     out = {
         "metric": METRIC,
         "value": round(p_s / l_s, 3) if (p_s and l_s) else None,
@@ -371,8 +392,10 @@ def bench_c5(args, pkg, cfg):
         "config": {"workload": f"c5: learned in-scattering, {args.c5_clouds} seeded 128^3 clouds of c1 statistics, "
                                f"{args.c5_target_spp} continuations per training point, evaluation {W}x{H} on a held-out "
                                f"cloud against a {args.c5_ref_spp}-spp plain render"},
-        "plain": {"spp": p_spp, "seconds": p_s, "curve": [[a, round(b, 4), round(c, 6)] for a, b, c in plain]},
-        "learned": {"spp": l_spp, "seconds": l_s, "curve": [[a, round(b, 4), round(c, 6)] for a, b, c in learned]},
+        "plain": {"spp": p_spp, "seconds": p_s, "relMSE": p_e,
+                  "progressive_curve_spp_s_relmse": [[a, round(b, 4), round(c, 6)] for a, b, c in plain]},
+        "learned": {"spp": l_spp, "seconds": l_s, "relMSE": l_e,
+                    "progressive_curve_spp_s_relmse": [[a, round(b, 4), round(c, 6)] for a, b, c in learned]},
         "training": {"points": n_points, "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
                      "steps": args.c5_train_steps, "batch": args.c5_batch, "final_loss": loss},
         "reference_s": round(t_ref, 2),
