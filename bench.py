@@ -44,7 +44,11 @@ def parse():
     ap.add_argument("--c5-clouds", type=int, default=200)
     ap.add_argument("--c5-points", type=int, default=80000, help="training points per cloud")
     ap.add_argument("--c5-target-spp", type=int, default=1024, help="continuations per training point")
-    ap.add_argument("--c5-train-steps", type=int, default=8000)
+    ap.add_argument("--c5-train-steps", type=int, default=40000)
+    ap.add_argument("--c5-collect-frames", type=int, default=8, help="record-pass frames per training cloud")
+    ap.add_argument("--c5-tracking", default="dda", choices=["dda", "global"],
+                    help="dda: brick-majorant tracking + ratio-tracked shadow rays (the production tracker); "
+                         "global: the reference's global majorant + shadow-volume NEE (c1's settings)")
     ap.add_argument("--c5-batch", type=int, default=4096)
     ap.add_argument("--c5-ref-spp", type=int, default=16384, help="frames of the converged plain reference")
     ap.add_argument("--c5-width", type=int, default=1920)
@@ -276,10 +280,15 @@ def bench_c5(args, pkg, cfg):
     ctx = pkg.Context(0)
     rng = np.random.default_rng(5)
 
+    S = pkg.structs
+
     def cloud_scene(seed, light, w=256, h=256):
         sc = cfg.c1(width=w, height=h)
         sc.fbm.seed = seed
         sc.light_dir = light
+        if args.c5_tracking == "dda":
+            sc.ext.trackingMode = S.CR_TRACK_BRICK_DDA
+            sc.ext.neeMode = S.CR_NEE_RATIO_TRACKING
         return sc
 
     def sun():
@@ -294,7 +303,7 @@ def bench_c5(args, pkg, cfg):
     for k in range(args.c5_clouds):
         sc = cloud_scene(1000 + k, sun())
         sc.setup(ctx)
-        ctx.learn_collect(cfg.frames(3, srand_seed=1 + k), args.c5_target_spp, args.c5_points)
+        ctx.learn_collect(cfg.frames(args.c5_collect_frames, srand_seed=1 + k), args.c5_target_spp, args.c5_points)
         if (k + 1) % 20 == 0:
             log(f"c5: {k + 1} clouds collected, {ctx.learn_dataset_size()} points, {time.perf_counter() - t0:.1f}s")
     ctx.synchronize()
@@ -303,9 +312,12 @@ def bench_c5(args, pkg, cfg):
     t0 = time.perf_counter()
     ctx.mlp_init(11)
     loss = None
-    for s in range(0, args.c5_train_steps, 1000):
-        loss = ctx.learn_train(min(1000, args.c5_train_steps - s), args.c5_batch, 1e-3 if s < args.c5_train_steps // 2 else 3e-4, seed=3)
-        log(f"c5: {s + 1000} training steps, loss {loss:.4g}")
+    n_steps = args.c5_train_steps
+    for s in range(0, n_steps, 1000):
+        lr = 1e-3 if s < n_steps // 3 else (3e-4 if s < 2 * n_steps // 3 else 1e-4)
+        loss = ctx.learn_train(min(1000, n_steps - s), args.c5_batch, lr, seed=3)
+        if (s // 1000) % 5 == 4:
+            log(f"c5: {s + 1000} training steps, loss {loss:.4g}")
     ctx.synchronize()
     t_train = time.perf_counter() - t0
 
@@ -391,7 +403,9 @@ def bench_c5(args, pkg, cfg):
         "data": "synthetic",
         "config": {"workload": f"c5: learned in-scattering, {args.c5_clouds} seeded 128^3 clouds of c1 statistics, "
                                f"{args.c5_target_spp} continuations per training point, evaluation {W}x{H} on a held-out "
-                               f"cloud against a {args.c5_ref_spp}-spp plain render"},
+                               f"cloud against a {args.c5_ref_spp}-spp plain render",
+                   "tracking": "brick-majorant DDA + ratio-tracked shadow rays" if args.c5_tracking == "dda"
+                   else "global majorant + shadow-volume NEE (the reference's)"},
         "plain": {"spp": p_spp, "seconds": p_s, "relMSE": p_e,
                   "progressive_curve_spp_s_relmse": [[a, round(b, 4), round(c, 6)] for a, b, c in plain]},
         "learned": {"spp": l_spp, "seconds": l_s, "relMSE": l_e,
