@@ -59,6 +59,9 @@ struct PTArgs {
     const float* points;    // continuation pass: records to continue
     float4* out;            // continuation pass: mean radiance per record
     uint32_t spp;           // continuation pass: samples per record
+    // scheduling (render pass): claim order of the tiles, and per-tile work of this launch
+    const uint32_t* tileOrder;  // tile for each queue slot, or null (raster order)
+    uint32_t* tileCost;         // lookups + brick visits per tile, accumulated, or null
 };
 
 // One persistent launch over several jobs: the job table (PTArgs per job,
@@ -78,6 +81,12 @@ int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee);
 hipError_t launch_pt_batch(const BatchArgs& b, int dtype, int layout, uint32_t tracking, uint32_t nee, int grid_blocks,
                            int block_threads, hipStream_t s);
 int pt_batch_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, int block_threads, uint32_t n_jobs);
+// Tile claim order for the next render: tiles by decreasing cost (a stable
+// radix sort of ~cost; ties keep raster order).  keys/vals/tmp: n words each,
+// counts: the radix sort's histograms (pt_order_counts(n) bytes).
+uint32_t pt_order_counts(uint32_t n);
+hipError_t launch_tile_order(const uint32_t* cost, uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
+                             uint32_t* vals_tmp, uint32_t* counts, hipStream_t s);
 // kern_pt_learn.hip
 hipError_t launch_pt_record(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s);
 hipError_t launch_pt_continue(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s);

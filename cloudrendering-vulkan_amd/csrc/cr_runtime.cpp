@@ -135,6 +135,7 @@ cr_status cr_destroy(cr_context* ctx)
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     void* bufs[] = {ctx->grid.data, ctx->grid.majorants, ctx->grid.skip, ctx->sv, ctx->image, ctx->dFrames, ctx->dQueue,
+                    ctx->tileCost, ctx->tileSort,
                     ctx->dCounters, ctx->dScalar};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -723,6 +724,30 @@ cr_status cr_pt_render(cr_context* ctx, const cr_push_constants* frames, uint32_
     if ((uint32_t)blocks > needed) blocks = (int)needed;
     if (ctx->ptBlocksOverride > 0) blocks = ctx->ptBlocksOverride;
     CR_HIP(ctx, zero_async(ctx->dQueue, 4, ctx->stream), "zero queue");
+    // Claim tiles in decreasing order of the previous render's per-tile work
+    // (same image shape), so the persistent grid drains on cheap tiles.  Only
+    // the schedule changes; every pixel's arithmetic is the same.
+    const uint32_t nTiles = a.totalIdx / 64u;
+    if (ctx->tileN != nTiles || ctx->tileLog2W != a.tileLog2W || !ctx->tileCost) {
+        if (ctx->tileCost) hipFree(ctx->tileCost);
+        ctx->tileCost = nullptr;
+        ctx->tileCostValid = false;
+        CR_HIP(ctx, hipMalloc((void**)&ctx->tileCost, (size_t)nTiles * 4), "hipMalloc tile costs");
+        ctx->tileN = nTiles;
+        ctx->tileLog2W = a.tileLog2W;
+    }
+    if (ctx->tileCostValid) {
+        st = ensure_buffer(ctx, (void**)&ctx->tileSort, &ctx->tileSortCap,
+                           (size_t)nTiles * 16 + pt_order_counts(nTiles), "tile order");
+        if (st != CR_OK) return st;
+        uint32_t* k = ctx->tileSort;
+        CR_HIP(ctx, launch_tile_order(ctx->tileCost, nTiles, k, k + nTiles, k + 2 * (size_t)nTiles, k + 3 * (size_t)nTiles,
+                                      k + 4 * (size_t)nTiles, ctx->stream), "tile order");
+        a.tileOrder = k + nTiles;
+    }
+    CR_HIP(ctx, zero_async(ctx->tileCost, (size_t)nTiles * 4, ctx->stream), "zero tile costs");
+    a.tileCost = ctx->tileCost;
+    ctx->tileCostValid = true;
     if (ctx->ptEventsUsed >= 1024) {
         st = fold_pt_events(ctx);
         if (st != CR_OK) return st;

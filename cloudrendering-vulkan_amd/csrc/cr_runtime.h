@@ -63,6 +63,12 @@ struct cr_context {
     double ptTotalMs = 0.0;
     uint64_t ptLaunches = 0;
     int ptBlocksOverride = 0;
+    // cost-ordered tile scheduling: per-tile work of the last render, and the sort buffers
+    uint32_t* tileCost = nullptr;
+    uint32_t* tileSort = nullptr;   // keys, vals, keys_tmp, vals_tmp (nTiles each) + histograms
+    size_t tileSortCap = 0;
+    uint32_t tileN = 0, tileLog2W = 0;
+    bool tileCostValid = false;
     uint32_t ptTileLog2W = 3;                  // 8x8 pixel tiles
 
     // photon beams / LBVH (cr_runtime_ppb.cpp)

@@ -1,7 +1,31 @@
 // Launches of the path tracer: the single-job render and the batched render.
 #include "pt_kernel.h"
 
+#include "cr_internal_ppb.h"
+
+namespace {
+
+__global__ void __launch_bounds__(256) order_keys_kernel(const uint32_t* cost, uint32_t n, uint32_t* keys, uint32_t* vals)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = ~cost[i];
+    vals[i] = i;
+}
+
+} // namespace
+
 namespace cr {
+
+uint32_t pt_order_counts(uint32_t n) { return ((n + 255u) / 256u + 1u) * 256u * 4u; }
+
+hipError_t launch_tile_order(const uint32_t* cost, uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
+                             uint32_t* vals_tmp, uint32_t* counts, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(order_keys_kernel, dim3((n + 255) / 256), dim3(256), 0, s, cost, n, keys, vals);
+    return launch_radix_sort(keys, vals, keys_tmp, vals_tmp, counts, n, s);
+}
 
 hipError_t launch_pt(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s)
 {

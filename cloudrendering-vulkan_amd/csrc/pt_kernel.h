@@ -45,6 +45,7 @@ enum : uint32_t { MODE_PRIMARY = 0, MODE_SHADOW = 1 };
 
 struct Lane {
     uint32_t job;  // batched launch: index into the job table
+    uint32_t cost0;  // the lane's work counters when it took its pixel
     int32_t px, py;
     uint32_t id;
     uint32_t f;
@@ -404,7 +405,10 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                     const uint32_t j = BATCH ? find_job(jobs, b.nJobs, idx) : 0u;
                     const cr::PTArgs& J = BATCH ? jobs[j] : a0;
                     const uint32_t local = BATCH ? idx - J.base : idx;
-                    const uint32_t tile = local >> 6, within = local & 63u;
+                    const uint32_t slot = local >> 6, within = local & 63u;
+                    // tiles in the order of the previous launch's cost, dearest first, so the
+                    // queue ends on cheap tiles and the last lanes finish together
+                    const uint32_t tile = J.tileOrder ? J.tileOrder[slot] : slot;
                     const uint32_t tl = J.tileLog2W, tw = 1u << tl;
                     const int32_t x = (int32_t)((tile % J.tilesX) * tw + (within & (tw - 1u)));
                     const int32_t y = (int32_t)((tile / J.tilesX) * (64u >> tl) + (within >> tl));
@@ -418,6 +422,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                             L.acc.x = v.x; L.acc.y = v.y; L.acc.z = v.z; L.acc.w = v.w;
                         }
                         L.f = 0;
+                        L.cost0 = c.lookups + c.brickVisits;
                         state = ST_ACTIVE;
                         // start frame 0: handled by the event loop below
                         L.mode = 0xFFu;  // marker: needs a new sample
@@ -455,6 +460,11 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                         float4 o;
                         o.x = L.acc.x; o.y = L.acc.y; o.z = L.acc.z; o.w = L.acc.w;
                         a.image[(size_t)L.py * a.W + L.px] = o;
+                        if (a.tileCost) {
+                            const uint32_t tl = a.tileLog2W;
+                            const uint32_t tile = ((uint32_t)L.py >> (6u - tl)) * a.tilesX + ((uint32_t)L.px >> tl);
+                            atomicAdd(a.tileCost + tile, (c.lookups + c.brickVisits) - L.cost0);
+                        }
                     } else if (KMODE == K_CONTINUE) {
                         const float n = (float)a.spp;
                         a.out[L.px] = make_float4(fdiv(L.acc.x, n), fdiv(L.acc.y, n), fdiv(L.acc.z, n), fdiv(L.acc.w, n));
