@@ -293,9 +293,10 @@ __device__ __forceinline__ f3 sv_norm(const cr_shadow_volume_properties& s, f3 p
 }
 
 // intersectCloud — PathTracer.comp:154-184.
-__device__ __forceinline__ bool intersect_box(const float* b0, const float* b1, f3 pos, f3 dir, float& tmin, float& tmax)
+// ... with the reciprocal direction given (invdir = 1 / dir).
+__device__ __forceinline__ bool intersect_box_inv(const float* b0, const float* b1, f3 pos, f3 inv, float& tmin,
+                                                  float& tmax)
 {
-    f3 inv = mk(fdiv(1.0f, dir.x), fdiv(1.0f, dir.y), fdiv(1.0f, dir.z));
     bool sx = inv.x < 0.0f, sy = inv.y < 0.0f, sz = inv.z < 0.0f;
     tmin = ((sx ? b1[0] : b0[0]) - pos.x) * inv.x;
     tmax = ((sx ? b0[0] : b1[0]) - pos.x) * inv.x;
@@ -310,6 +311,13 @@ __device__ __forceinline__ bool intersect_box(const float* b0, const float* b1, 
     if (tzmin > tmin) tmin = tzmin;
     if (tzmax < tmax) tmax = tzmax;
     return true;
+}
+
+__device__ __forceinline__ f3 recip(f3 d) { return mk(fdiv(1.0f, d.x), fdiv(1.0f, d.y), fdiv(1.0f, d.z)); }
+
+__device__ __forceinline__ bool intersect_box(const float* b0, const float* b1, f3 pos, f3 dir, float& tmin, float& tmax)
+{
+    return intersect_box_inv(b0, b1, pos, recip(dir), tmin, tmax);
 }
 
 // isInCloud — ShadowVolume.comp:34-42.

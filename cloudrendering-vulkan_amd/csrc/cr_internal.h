@@ -31,6 +31,7 @@ struct PTArgs {
     const float* skip;      // majorants with empty bricks replaced by -(distance to a non-empty brick)
     int32_t mx, my, mz;
     float bw[3];
+    float ibw[3];           // 1 / bw
     // shadow volume
     const float* sv;
     int32_t svx, svy, svz;

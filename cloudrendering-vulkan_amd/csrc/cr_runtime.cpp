@@ -668,7 +668,10 @@ void fill_job_args(PTArgs& a, const DeviceGrid& g, const float* sv, const uint32
     a.grid = g.data; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nbx = g.nbx; a.nby = g.nby;
     a.majorants = g.majorants; a.skip = g.skip; a.mx = g.mx; a.my = g.my; a.mz = g.mz;
     const int32_t dims[3] = {g.nx, g.ny, g.nz};
-    for (int k = 0; k < 3; k++) a.bw[k] = ((cloud.bounds[1][k] - cloud.bounds[0][k]) * (float)g.mb) / (float)dims[k];
+    for (int k = 0; k < 3; k++) {
+        a.bw[k] = ((cloud.bounds[1][k] - cloud.bounds[0][k]) * (float)g.mb) / (float)dims[k];
+        a.ibw[k] = 1.0f / a.bw[k];
+    }
     a.sv = sv;
     a.svx = (int32_t)svDims[0]; a.svy = (int32_t)svDims[1]; a.svz = (int32_t)svDims[2];
     a.cam = cam;

@@ -97,24 +97,27 @@ __device__ __forceinline__ float* tdelta_lds()
     return td;
 }
 
-__device__ __forceinline__ void dda_init(const cr::PTArgs& a, f3 pos, f3 dir, Lane& L)
+// The walk's set-up multiplies by the reciprocals the box test already formed
+// (inv = 1 / dir) and by 1 / bw (a.ibw), instead of dividing.
+__device__ __forceinline__ void dda_init(const cr::PTArgs& a, f3 pos, f3 dir, f3 invd, Lane& L)
 {
     const float p[3] = {pos.x, pos.y, pos.z};
     const float v[3] = {dir.x, dir.y, dir.z};
+    const float iv[3] = {invd.x, invd.y, invd.z};
     const int32_t mc[3] = {a.mx, a.my, a.mz};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        float rel = fdiv(p[k] - B0(a)[k], a.bw[k]);
+        float rel = (p[k] - B0(a)[k]) * a.ibw[k];
         int32_t c = (int32_t)floorf(fminf(fmaxf(rel, 0.0f), (float)mc[k]));
         c = clampi(c, 0, mc[k] - 1);
         L.cell[k] = c;
         float td;
         if (v[k] > 0.0f) {
-            L.tNext[k] = fmaxf(fdiv((B0(a)[k] + (float)(c + 1) * a.bw[k]) - p[k], v[k]), 0.0f);
-            td = fdiv(a.bw[k], v[k]);
+            L.tNext[k] = fmaxf(((B0(a)[k] + (float)(c + 1) * a.bw[k]) - p[k]) * iv[k], 0.0f);
+            td = a.bw[k] * iv[k];
         } else if (v[k] < 0.0f) {
-            L.tNext[k] = fmaxf(fdiv((B0(a)[k] + (float)c * a.bw[k]) - p[k], v[k]), 0.0f);
-            td = fdiv(a.bw[k], -v[k]);
+            L.tNext[k] = fmaxf(((B0(a)[k] + (float)c * a.bw[k]) - p[k]) * iv[k], 0.0f);
+            td = a.bw[k] * -iv[k];
         } else {
             L.tNext[k] = __builtin_inff();
             td = __builtin_inff();
@@ -165,12 +168,13 @@ template <int TRACK>
 __device__ __forceinline__ bool begin_segment(const cr::PTArgs& a, Lane& L, f3 dir)
 {
     float tmax = 0.0f, tmin = 0.0f;
-    if (!intersect_box(B0(a), B1(a), L.pos, dir, tmin, tmax)) return false;
+    const f3 invd = recip(dir);
+    if (!intersect_box_inv(B0(a), B1(a), L.pos, invd, tmin, tmax)) return false;
     f3 exitp = add(L.pos, mul(dir, tmax));
     L.dist = length(sub(L.pos, exitp));
     L.t = 0.0f;
     if (TRACK == CR_TRACK_BRICK_DDA) {
-        dda_init(a, L.pos, dir, L);
+        dda_init(a, L.pos, dir, invd, L);
         L.tau = -crd_logf(xs_float(L.rng));
     }
     return true;

@@ -133,9 +133,9 @@ static inline f3 sv_norm(const cr_shadow_volume_properties& s, f3 p)
 }
 
 /* intersectCloud — PathTracer.comp:154-184 (scratchapixel slab test). */
-static inline bool intersect_box(const float* b0, const float* b1, f3 pos, f3 dir, float& tmin, float& tmax)
+/* intersectCloud with the reciprocal direction given (invdir = 1 / dir). */
+static inline bool intersect_box_inv(const float* b0, const float* b1, f3 pos, f3 inv, float& tmin, float& tmax)
 {
-    f3 inv = mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
     const float* B[2] = {b0, b1};
     int sx = inv.x < 0.0f, sy = inv.y < 0.0f, sz = inv.z < 0.0f;
     tmin = (B[sx][0] - pos.x) * inv.x;
@@ -151,6 +151,13 @@ static inline bool intersect_box(const float* b0, const float* b1, f3 pos, f3 di
     if (tzmin > tmin) tmin = tzmin;
     if (tzmax < tmax) tmax = tzmax;
     return true;
+}
+
+static inline f3 recip(f3 d) { return mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); }
+
+static inline bool intersect_box(const float* b0, const float* b1, f3 pos, f3 dir, float& tmin, float& tmax)
+{
+    return intersect_box_inv(b0, b1, pos, recip(dir), tmin, tmax);
 }
 
 /* isInCloud — ShadowVolume.comp:34-42 */

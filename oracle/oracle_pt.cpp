@@ -177,6 +177,7 @@ struct Ctx {
     float extScale;   /* densityScaling / baseScaling is never formed: see findScatterPoint */
     int32_t mx, my, mz;
     float bw[3];      /* world size of a majorant brick per axis */
+    float ibw[3];     /* 1 / bw */
     or_counters* cnt;
 };
 
@@ -193,24 +194,27 @@ struct Dda {
     float tDelta[3];
 };
 
-inline void dda_init(const Ctx& C, f3 pos, f3 dir, Dda& d)
+/* The walk's set-up multiplies by the reciprocals the box test already
+ * formed (inv = 1 / dir) and by 1 / bw, instead of dividing. */
+inline void dda_init(const Ctx& C, f3 pos, f3 dir, f3 invd, Dda& d)
 {
     const float p[3] = {pos.x, pos.y, pos.z};
     const float v[3] = {dir.x, dir.y, dir.z};
+    const float iv[3] = {invd.x, invd.y, invd.z};
     const int32_t mc[3] = {C.mx, C.my, C.mz};
     for (int k = 0; k < 3; k++) {
-        float rel = (p[k] - C.b0[k]) / C.bw[k];
+        float rel = (p[k] - C.b0[k]) * C.ibw[k];
         int32_t c = (int32_t)floorf(fminf(fmaxf(rel, 0.0f), (float)mc[k]));
         c = clampi(c, 0, mc[k] - 1);
         d.cell[k] = c;
         if (v[k] > 0.0f) {
             d.step[k] = 1;
-            d.tNext[k] = fmaxf(((C.b0[k] + (float)(c + 1) * C.bw[k]) - p[k]) / v[k], 0.0f);
-            d.tDelta[k] = C.bw[k] / v[k];
+            d.tNext[k] = fmaxf(((C.b0[k] + (float)(c + 1) * C.bw[k]) - p[k]) * iv[k], 0.0f);
+            d.tDelta[k] = C.bw[k] * iv[k];
         } else if (v[k] < 0.0f) {
             d.step[k] = -1;
-            d.tNext[k] = fmaxf(((C.b0[k] + (float)c * C.bw[k]) - p[k]) / v[k], 0.0f);
-            d.tDelta[k] = C.bw[k] / -v[k];
+            d.tNext[k] = fmaxf(((C.b0[k] + (float)c * C.bw[k]) - p[k]) * iv[k], 0.0f);
+            d.tDelta[k] = C.bw[k] * -iv[k];
         } else {
             d.step[k] = 0;
             d.tNext[k] = INFINITY;
@@ -247,13 +251,14 @@ inline float brick_majorant(const Ctx& C, const Dda& d)
 inline bool find_scatter(const Ctx& C, f3& pos, f3 dir, uint32_t& rng)
 {
     float tmax = 0.0f, tmin = 0.0f;
-    if (!intersect_box(C.b0, C.b1, pos, dir, tmin, tmax)) return false;
+    const f3 invd = recip(dir);
+    if (!intersect_box_inv(C.b0, C.b1, pos, invd, tmin, tmax)) return false;
     f3 exitp = add(pos, mul(dir, tmax));
     float dist = length(sub(pos, exitp));
     const cr_cloud_properties& cp = C.s->cloud;
     if (C.s->ext.trackingMode == CR_TRACK_BRICK_DDA) {
         Dda d;
-        dda_init(C, pos, dir, d);
+        dda_init(C, pos, dir, invd, d);
         float t = 0.0f;
         float tau = -o_logf(xs_float(rng));
         for (;;) {
@@ -301,14 +306,15 @@ inline bool find_scatter(const Ctx& C, f3& pos, f3 dir, uint32_t& rng)
 inline float ratio_track(const Ctx& C, f3 pos, f3 dir, uint32_t& rng)
 {
     float tmax = 0.0f, tmin = 0.0f;
-    if (!intersect_box(C.b0, C.b1, pos, dir, tmin, tmax)) return 1.0f;
+    const f3 invd = recip(dir);
+    if (!intersect_box_inv(C.b0, C.b1, pos, invd, tmin, tmax)) return 1.0f;
     f3 exitp = add(pos, mul(dir, tmax));
     float dist = length(sub(pos, exitp));
     const cr_cloud_properties& cp = C.s->cloud;
     float T = 1.0f;
     if (C.s->ext.trackingMode == CR_TRACK_BRICK_DDA) {
         Dda d;
-        dda_init(C, pos, dir, d);
+        dda_init(C, pos, dir, invd, d);
         float t = 0.0f;
         float tau = -o_logf(xs_float(rng));
         for (;;) {
@@ -371,8 +377,11 @@ void init_ctx(Ctx& C, const or_scene* s, or_counters* cnt)
     C.my = (int32_t)s->mdims[1];
     C.mz = (int32_t)s->mdims[2];
     for (int k = 0; k < 3; k++)
+    {
         C.bw[k] = ((s->cloud.bounds[1][k] - s->cloud.bounds[0][k]) * (float)(s->ext.majorantBrick ? s->ext.majorantBrick : 16u)) /
                   (float)s->dims[k];
+        C.ibw[k] = 1.0f / C.bw[k];
+    }
     C.cnt = cnt;
 }
 
