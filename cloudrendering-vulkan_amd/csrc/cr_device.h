@@ -8,8 +8,9 @@
 // binary32 implementation that evaluates the same sequence.  The reference
 // calls GLSL log/exp/sin/cos/pow, whose precision the Vulkan driver defines;
 // the functions below are accurate to about one ULP:
-//   crd_logf    : x = m 2^e, m in [sqrt(1/2), sqrt(2)], s = (m-1)/(m+1),
-//                 log m = 2 atanh(s) by its series to s^11 (Cody & Waite 1980),
+//   crd_logf    : x = m 2^e, m in [sqrt(1/2), sqrt(2)], f = m - 1,
+//                 log m = f - f^2/2 + f^3 P(f), P of degree 7 fitted to the
+//                 relative error on that range (no division),
 //                 log x = e ln2 + log m with ln2 = hi + lo.
 //   crd_expf    : Cody-Waite reduction x = k ln2 + r, Taylor series to r^8.
 //   crd_sincosf : reduction by pi/2 in three parts, Taylor series of sin to
@@ -82,16 +83,17 @@ __device__ __forceinline__ float crd_logf(float x)
     e += (int32_t)((b >> 23) & 0xffu) - 127;
     float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
     if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
-    float s = fdiv(m - 1.0f, m + 1.0f);
-    float z = s * s;
-    float p = 0.0909090909f;
-    p = p * z + 0.111111111f;
-    p = p * z + 0.142857143f;
-    p = p * z + 0.2f;
-    p = p * z + 0.333333333f;
-    p = p * z;
-    float s2 = s + s;
-    float logm = s2 + s2 * p;
+    float f = m - 1.0f;  // exact: m is within a factor 2 of 1
+    float z = f * f;
+    float p = -0.07369332015514374f;
+    p = p * f + 0.12623557448387146f;
+    p = p * f + -0.1320105493068695f;
+    p = p * f + 0.14226537942886353f;
+    p = p * f + -0.166214257478714f;
+    p = p * f + 0.19999857246875763f;
+    p = p * f + -0.2500085234642029f;
+    p = p * f + 0.3333335220813751f;
+    float logm = (f - 0.5f * z) + (z * f) * p;
     float fe = (float)e;
     return fe * 0.693145751953125f + (logm + fe * 1.42860682030941723212e-6f);
 }

@@ -10,11 +10,13 @@
  * faithful restatement.  Each function below is pinned against double
  * precision libm in tests/test_oracle.py.
  *
- *  o_logf   : x = m * 2^e with m in [sqrt(1/2), sqrt(2)], s = (m-1)/(m+1),
- *             log m = 2 atanh s = 2s (1 + s^2/3 + s^4/5 + ... + s^10/11)
- *             (the atanh series used by Cody & Waite, "Software Manual for
- *             the Elementary Functions", 1980), log x = e ln2 + log m with
- *             ln2 split in a high and a low part.
+ *  o_logf   : x = m * 2^e with m in [sqrt(1/2), sqrt(2)], f = m - 1,
+ *             log m = f - f^2/2 + f^3 P(f) with P of degree 7, its
+ *             coefficients fitted by the build (least squares weighted to the
+ *             relative error of log m, refined by Lawson iterations toward
+ *             the minimax fit, on f in [sqrt(1/2)-1, sqrt(2)-1]; no
+ *             division), log x = e ln2 + log m with ln2 split in a high and a
+ *             low part (Cody & Waite's reduction); <= 1.3 ULP.
  *  o_expf   : x = k ln2 + r, |r| <= ln2/2 (Cody-Waite reduction), Taylor
  *             series of e^r to r^8, scaled by 2^k.
  *  o_sincosf: quadrant reduction by pi/2 in three parts (Cody-Waite),
@@ -51,16 +53,17 @@ static inline float o_logf(float x)
     e += (int32_t)((b >> 23) & 0xffu) - 127;
     float m = o_u2f((b & 0x007fffffu) | 0x3f800000u); /* [1,2) */
     if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
-    float s = (m - 1.0f) / (m + 1.0f);
-    float z = s * s;
-    float p = 0.0909090909f;          /* 1/11 */
-    p = p * z + 0.111111111f;         /* 1/9  */
-    p = p * z + 0.142857143f;         /* 1/7  */
-    p = p * z + 0.2f;                 /* 1/5  */
-    p = p * z + 0.333333333f;         /* 1/3  */
-    p = p * z;
-    float s2 = s + s;
-    float logm = s2 + s2 * p;
+    float f = m - 1.0f;  /* exact: m is within a factor 2 of 1 */
+    float z = f * f;
+    float p = -0.07369332015514374f;
+    p = p * f + 0.12623557448387146f;
+    p = p * f + -0.1320105493068695f;
+    p = p * f + 0.14226537942886353f;
+    p = p * f + -0.166214257478714f;
+    p = p * f + 0.19999857246875763f;
+    p = p * f + -0.2500085234642029f;
+    p = p * f + 0.3333335220813751f;
+    float logm = (f - 0.5f * z) + (z * f) * p;
     float fe = (float)e;
     return fe * 0.693145751953125f + (logm + fe * 1.42860682030941723212e-6f);
 }
