@@ -18,7 +18,8 @@ struct DeviceGrid {
     int32_t nbx = 0, nby = 0, nbz = 0;
     size_t bytes = 0;
     float* majorants = nullptr; // ceil(N/mb)^3 brick maxima (with apron)
-    float* skip = nullptr;      // (majorant * ds) / bs; empty bricks -(Chebyshev distance to a non-empty one, <= 16)
+    float* skip = nullptr;      // per brick: (majorant * ds) / bs, or for an empty brick -(Chebyshev distance to a
+                                // non-empty one, <= 16); then the majorant
     int32_t mx = 0, my = 0, mz = 0;
     int32_t mb = 16;            // voxels per side of a majorant brick
 };
@@ -27,8 +28,7 @@ struct PTArgs {
     // density
     const void* grid;
     int32_t nx, ny, nz, nbx, nby;
-    const float* majorants;
-    const float* skip;      // majorants with empty bricks replaced by -(distance to a non-empty brick)
+    const float* skip;      // (scaled majorant or -(distance to a non-empty brick), majorant) per brick
     int32_t mx, my, mz;
     float bw[3];
     float ibw[3];           // 1 / bw

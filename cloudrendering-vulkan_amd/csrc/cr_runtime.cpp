@@ -220,6 +220,8 @@ cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb)
 {
     int32_t mx = (g.nx + mb - 1) / mb, my = (g.ny + mb - 1) / mb, mz = (g.nz + mb - 1) / mb;
     const size_t n = (size_t)mx * my * mz;
+    // the brick walk addresses the two-float skip table with 32-bit byte offsets
+    if (n >= ((size_t)1 << 29)) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "more than 2^29 majorant bricks");
     if (g.majorants && (g.mx != mx || g.my != my || g.mz != mz)) {
         hipFree(g.majorants);
         hipFree(g.skip);
@@ -228,7 +230,7 @@ cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb)
     }
     if (!g.majorants) {
         hipError_t e = hipMalloc((void**)&g.majorants, n * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc((void**)&g.skip, n * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void**)&g.skip, 2 * n * sizeof(float));
         if (e != hipSuccess) {
             if (g.majorants) hipFree(g.majorants);
             g.majorants = nullptr;
@@ -376,7 +378,7 @@ cr_status cr_grid_skip(cr_context* ctx, float* out)
     if (!ctx->hasGrid) return fail(ctx, CR_ERR_STATE, "no grid");
     if (!out) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null output");
     size_t n = (size_t)ctx->grid.mx * ctx->grid.my * ctx->grid.mz;
-    CR_HIP(ctx, hipMemcpyAsync(out, ctx->grid.skip, n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy skip grid");
+    CR_HIP(ctx, hipMemcpy2DAsync(out, 4, ctx->grid.skip, 8, 4, n, hipMemcpyDeviceToHost, ctx->stream), "copy skip grid");
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return CR_OK;
 }
@@ -666,7 +668,7 @@ void fill_job_args(PTArgs& a, const DeviceGrid& g, const float* sv, const uint32
 {
     memset(&a, 0, sizeof(a));
     a.grid = g.data; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nbx = g.nbx; a.nby = g.nby;
-    a.majorants = g.majorants; a.skip = g.skip; a.mx = g.mx; a.my = g.my; a.mz = g.mz;
+    a.skip = g.skip; a.mx = g.mx; a.my = g.my; a.mz = g.mz;
     const int32_t dims[3] = {g.nx, g.ny, g.nz};
     for (int k = 0; k < 3; k++) {
         a.bw[k] = ((cloud.bounds[1][k] - cloud.bounds[0][k]) * (float)g.mb) / (float)dims[k];

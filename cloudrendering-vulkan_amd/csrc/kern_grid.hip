@@ -180,7 +180,9 @@ __global__ void __launch_bounds__(256) majorant_kernel(const T* g, int32_t nx, i
 // pass); a brick first covered in round d is d bricks (Chebyshev) from the
 // nearest non-empty brick.  Bricks still uncovered keep -16.
 // A non-empty brick stores its extinction majorant (m * densityScaling) /
-// baseScaling, the value the brick walk needs.
+// baseScaling, the value the brick walk needs.  Each brick holds two floats,
+// that value and the majorant m itself, so the walk's one 8-byte fetch per
+// brick also yields the m its density test divides by.
 __global__ void __launch_bounds__(256) skip_init_kernel(const float* maj, uint32_t n, uint8_t* occ, float* skip, float ds,
                                                         float bs)
 {
@@ -188,7 +190,8 @@ __global__ void __launch_bounds__(256) skip_init_kernel(const float* maj, uint32
     if (i >= n) return;
     const float m = maj[i];
     occ[i] = m > 0.0f ? 1 : 0;
-    skip[i] = m > 0.0f ? fdiv(m * ds, bs) : -16.0f;
+    skip[2 * i] = m > 0.0f ? fdiv(m * ds, bs) : -16.0f;
+    skip[2 * i + 1] = m;
 }
 
 __global__ void __launch_bounds__(256) skip_scale_kernel(const float* maj, uint32_t n, float* skip, float ds, float bs)
@@ -196,7 +199,7 @@ __global__ void __launch_bounds__(256) skip_scale_kernel(const float* maj, uint3
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const float m = maj[i];
-    if (m > 0.0f) skip[i] = fdiv(m * ds, bs);
+    if (m > 0.0f) skip[2 * i] = fdiv(m * ds, bs);
 }
 
 __global__ void __launch_bounds__(256) dilate_kernel(const uint8_t* in, uint8_t* out, int32_t mx, int32_t my, int32_t mz,
@@ -214,7 +217,7 @@ __global__ void __launch_bounds__(256) dilate_kernel(const uint8_t* in, uint8_t*
     if (c > 0) v |= in[i - stride];
     if (c + 1 < N) v |= in[i + stride];
     out[i] = v;
-    if (skip && v && skip[i] == -16.0f) skip[i] = -d;  // last pass of round d
+    if (skip && v && skip[2 * i] == -16.0f) skip[2 * i] = -d;  // last pass of round d
 }
 
 // Majorants are >= 0, so their IEEE bit patterns order like unsigned integers.

@@ -62,6 +62,7 @@ struct Lane {
     float w;      // thr * albedo of the pending shadow ray
     // brick DDA
     int32_t cell[3];
+    uint32_t brick;  // cell[0] + mx (cell[1] + my cell[2])
     float tNext[3];
     float tau;
 };
@@ -88,13 +89,20 @@ __device__ __forceinline__ float sample_sv(const cr::PTArgs& a, f3 p)
     return trilinear<float, LAYOUT_DENSE>(g, sv_norm(a.svp, p));
 }
 
-// bw / |direction| per axis for the lane's current segment (computed once per
-// segment by dda_init with the same division every brick step would make),
-// kept in LDS to spare registers.
-__device__ __forceinline__ float* tdelta_lds()
+// Per lane and axis, for the lane's current segment (written once per segment
+// by dda_init), kept in LDS to spare registers: the step in t between brick
+// boundaries (bw / |direction|, the same quotient every brick step would
+// form) and the step in the linear brick index (+stride, -stride or 0; its
+// sign is the step in the cell index).  One 8-byte read per brick step.
+struct DdaAxis {
+    float td;
+    int32_t brickStep;
+};
+
+__device__ __forceinline__ DdaAxis* dda_lds()
 {
-    __shared__ float td[256 * 3];
-    return td;
+    __shared__ DdaAxis d[256 * 3];
+    return d;
 }
 
 // The walk's set-up multiplies by the reciprocals the box test already formed
@@ -105,61 +113,63 @@ __device__ __forceinline__ void dda_init(const cr::PTArgs& a, f3 pos, f3 dir, f3
     const float v[3] = {dir.x, dir.y, dir.z};
     const float iv[3] = {invd.x, invd.y, invd.z};
     const int32_t mc[3] = {a.mx, a.my, a.mz};
+    const int32_t stride[3] = {1, a.mx, a.mx * a.my};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         float rel = (p[k] - B0(a)[k]) * a.ibw[k];
         int32_t c = (int32_t)floorf(fminf(fmaxf(rel, 0.0f), (float)mc[k]));
         c = clampi(c, 0, mc[k] - 1);
         L.cell[k] = c;
-        float td;
+        DdaAxis d;
         if (v[k] > 0.0f) {
             L.tNext[k] = fmaxf(((B0(a)[k] + (float)(c + 1) * a.bw[k]) - p[k]) * iv[k], 0.0f);
-            td = a.bw[k] * iv[k];
+            d.td = a.bw[k] * iv[k];
+            d.brickStep = stride[k];
         } else if (v[k] < 0.0f) {
             L.tNext[k] = fmaxf(((B0(a)[k] + (float)c * a.bw[k]) - p[k]) * iv[k], 0.0f);
-            td = a.bw[k] * -iv[k];
+            d.td = a.bw[k] * -iv[k];
+            d.brickStep = -stride[k];
         } else {
             L.tNext[k] = __builtin_inff();
-            td = __builtin_inff();
+            d.td = __builtin_inff();
+            d.brickStep = 0;
         }
-        tdelta_lds()[threadIdx.x * 3 + k] = td;
+        dda_lds()[threadIdx.x * 3 + k] = d;
     }
-}
-
-template <int K>
-__device__ __forceinline__ void dda_step_axis(const cr::PTArgs& a, float v, int32_t m, Lane& L)
-{
-    int32_t c = L.cell[K] + (v > 0.0f ? 1 : (v < 0.0f ? -1 : 0));
-    L.cell[K] = c;
-    L.tNext[K] = L.tNext[K] + tdelta_lds()[threadIdx.x * 3 + K];
+    L.brick = (uint32_t)L.cell[0] + (uint32_t)a.mx * ((uint32_t)L.cell[1] + (uint32_t)a.my * (uint32_t)L.cell[2]);
 }
 
 // Advance to the neighbouring brick along the axis of the nearest boundary
 // (ties: x before y before z); false when the ray leaves the brick grid.
-__device__ __forceinline__ bool dda_advance(const cr::PTArgs& a, f3 dir, Lane& L)
+// Written with selects rather than branches: every lane of a wave takes the
+// same instructions whichever axis it steps.
+__device__ __forceinline__ bool dda_advance(const cr::PTArgs& a, Lane& L)
 {
-    if (L.tNext[0] <= L.tNext[1] && L.tNext[0] <= L.tNext[2]) {
-        dda_step_axis<0>(a, dir.x, a.mx, L);
-        return L.cell[0] >= 0 && L.cell[0] < a.mx;
-    }
-    if (L.tNext[1] <= L.tNext[2]) {
-        dda_step_axis<1>(a, dir.y, a.my, L);
-        return L.cell[1] >= 0 && L.cell[1] < a.my;
-    }
-    dda_step_axis<2>(a, dir.z, a.mz, L);
-    return L.cell[2] >= 0 && L.cell[2] < a.mz;
+    const bool sx = L.tNext[0] <= L.tNext[1] && L.tNext[0] <= L.tNext[2];
+    const bool sy = !sx && L.tNext[1] <= L.tNext[2];
+    const bool sz = !sx && !sy;
+    const DdaAxis d = dda_lds()[threadIdx.x * 3 + (sx ? 0 : (sy ? 1 : 2))];
+    const float tn = (sx ? L.tNext[0] : (sy ? L.tNext[1] : L.tNext[2])) + d.td;
+    const int32_t step = min(max(d.brickStep, -1), 1);
+    L.cell[0] += sx ? step : 0;
+    L.cell[1] += sy ? step : 0;
+    L.cell[2] += sz ? step : 0;
+    L.tNext[0] = sx ? tn : L.tNext[0];
+    L.tNext[1] = sy ? tn : L.tNext[1];
+    L.tNext[2] = sz ? tn : L.tNext[2];
+    L.brick += (uint32_t)d.brickStep;
+    return ((uint32_t)L.cell[0] < (uint32_t)a.mx) & ((uint32_t)L.cell[1] < (uint32_t)a.my) &
+           ((uint32_t)L.cell[2] < (uint32_t)a.mz);
 }
 
-__device__ __forceinline__ uint32_t brick_index(const cr::PTArgs& a, const Lane& L)
+// The skip grid, two floats per brick: (majorant * densityScaling) /
+// baseScaling of a non-empty brick, -d for an empty brick d bricks from the
+// nearest non-empty one; then the brick's majorant.  Brick tables hold < 2^29
+// entries, so the byte offset fits 32 bits.
+__device__ __forceinline__ float2 brick_skip(const cr::PTArgs& a, const Lane& L)
 {
-    return (uint32_t)L.cell[0] + (uint32_t)a.mx * ((uint32_t)L.cell[1] + (uint32_t)a.my * (uint32_t)L.cell[2]);
+    return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(a.skip) + (L.brick << 3));
 }
-
-__device__ __forceinline__ float brick_majorant(const cr::PTArgs& a, const Lane& L) { return a.majorants[brick_index(a, L)]; }
-
-// The skip grid: (majorant * densityScaling) / baseScaling of a non-empty
-// brick, -d for an empty brick d bricks from the nearest non-empty one.
-__device__ __forceinline__ float brick_skip(const cr::PTArgs& a, const Lane& L) { return a.skip[brick_index(a, L)]; }
 
 // Start a tracking segment from L.pos along `dir` (primary or shadow).
 // Returns false when the segment misses the box (no RNG draw is made then,
@@ -190,13 +200,17 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
     const bool shadow = L.mode == MODE_SHADOW;
     const f3 dir = shadow ? sdir : L.dir;
     if (TRACK == CR_TRACK_BRICK_DDA) {
-        float maj;
+        float maj = 0.0f;
         // Empty-space skipping: the skip grid holds -d for an empty brick whose
         // nearest non-empty brick is d bricks away (Chebyshev), so every brick
         // of the cube of radius d-1 around it is empty; while the walk stays in
         // that cube no majorant is fetched.  The arithmetic of the walk is
         // unchanged (an empty brick has majorant 0 either way).
-        int32_t ex = 0, ey = 0, ez = 0, er = -1;
+        // The cube is kept as its low corner and edge span: cell - low, taken
+        // unsigned, is <= span exactly inside it.  The initial corner 2^30 lies
+        // beyond every brick, so no brick is inside before the first fetch.
+        int32_t lx = 1 << 30, ly = 1 << 30, lz = 1 << 30;
+        uint32_t span = 0;
         // At most DDA_STEPS_PER_PASS bricks per call: a lane crossing a long
         // stretch of thin or empty bricks resumes in the next pass instead of
         // holding its wave's other lanes idle (the walk's state is in Lane).
@@ -205,15 +219,20 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
             // sb = (majorant * densityScaling) / baseScaling, precomputed per
             // brick in the skip grid (the same IEEE division as here)
             float sb;
-            if (abs(L.cell[0] - ex) <= er && abs(L.cell[1] - ey) <= er && abs(L.cell[2] - ez) <= er) {
+            const bool inside = ((uint32_t)(L.cell[0] - lx) <= span) & ((uint32_t)(L.cell[1] - ly) <= span) &
+                                ((uint32_t)(L.cell[2] - lz) <= span);
+            if (inside) {
                 sb = 0.0f;
             } else {
-                sb = brick_skip(a, L);
+                const float2 sk = brick_skip(a, L);
+                sb = sk.x;
+                maj = sk.y;
                 if (sb < 0.0f) {
-                    ex = L.cell[0];
-                    ey = L.cell[1];
-                    ez = L.cell[2];
-                    er = (int32_t)(-sb) - 1;
+                    const int32_t r = (int32_t)(-sb) - 1;
+                    lx = L.cell[0] - r;
+                    ly = L.cell[1] - r;
+                    lz = L.cell[2] - r;
+                    span = (uint32_t)(2 * r);
                     sb = 0.0f;
                 }
             }
@@ -227,9 +246,9 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
             L.tau = L.tau - sb * seg;
             L.t = tc;
             if (!(tc < L.dist)) return EV_EXIT;
-            if (!dda_advance(a, dir, L)) return EV_EXIT;
+            if (!dda_advance(a, L)) return EV_EXIT;
         }
-        maj = brick_majorant(a, L);
+        // the loop leaves only from a brick whose entry it fetched (sb > 0)
         f3 cur = add(L.pos, mul(dir, L.t));
         float e = sample_cloud<T, LAYOUT>(a, cur);
         c.lookups++;
