@@ -36,9 +36,15 @@ namespace {
 
 constexpr uint32_t COLLISION_CEILING = 1u << 20;  // guards a non-terminating walk only
 #ifndef PT_DDA_STEPS_PER_PASS
-#define PT_DDA_STEPS_PER_PASS 8
+#define PT_DDA_STEPS_PER_PASS 6
 #endif
 constexpr uint32_t DDA_STEPS_PER_PASS = PT_DDA_STEPS_PER_PASS;  // brick-walk steps per tracking call
+#ifndef PT_TAIL_MIN_LANES
+#define PT_TAIL_MIN_LANES 16
+#endif
+// The event tail runs in a pass only when at least this many lanes of the
+// wave (or all of its active lanes) have an event pending; the others wait.
+constexpr uint32_t TAIL_MIN_LANES = PT_TAIL_MIN_LANES;
 
 enum : uint32_t { ST_IDLE = 0, ST_ACTIVE = 1, ST_DEAD = 2 };
 enum : uint32_t { MODE_PRIMARY = 0, MODE_SHADOW = 1 };
@@ -402,6 +408,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
     const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     L.mode = MODE_PRIMARY;
     L.job = 0;
+    uint32_t ev = EV_NONE;  // the lane's pending event, resolved in the tail
 
     for (;;) {
         // ---- claim pixels for idle lanes (wave-wide, one atomic per wave)
@@ -423,6 +430,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                     L.acc.x = 0.0f; L.acc.y = 0.0f; L.acc.z = 0.0f; L.acc.w = 0.0f;
                     L.f = 0;
                     state = ST_ACTIVE;
+                    ev = EV_NONE;
                     L.mode = 0xFFu;
                 } else {
                     const uint32_t j = BATCH ? find_job(jobs, b.nJobs, idx) : 0u;
@@ -447,6 +455,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                         L.f = 0;
                         L.cost0 = c.lookups + c.brickVisits;
                         state = ST_ACTIVE;
+                        ev = EV_NONE;
                         // start frame 0: handled by the event loop below
                         L.mode = 0xFFu;  // marker: needs a new sample
                     }
@@ -463,16 +472,20 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
         const float albedo = a.ext.albedo;
         const uint32_t cap = a.ext.bounceCap;
 
-        uint32_t ev = EV_NONE;
-        if (L.mode != 0xFFu) ev = track_step<T, LAYOUT, TRACK>(a, L, sdir, c);
+        if (ev == EV_NONE && L.mode != 0xFFu) ev = track_step<T, LAYOUT, TRACK>(a, L, sdir, c);
 
         // ---- divergent tail: events that need no lookup are resolved here,
         // possibly several in a row (e.g. a camera ray that misses the box).
         // Each pass resolves one event per lane; the work several events end
         // with (sampling a new direction, starting a tracking segment) is done
         // once per pass for every lane that needs it, so lanes handling
-        // different events share those instructions.
+        // different events share those instructions.  A wave enters the tail
+        // once enough of its lanes have an event (TAIL_MIN_LANES), so its cost
+        // is shared by more lanes; until then those lanes keep their event.
         bool needSample = (L.mode == 0xFFu);
+        const uint32_t nWant = (uint32_t)__popcll(__ballot(ev != EV_NONE || needSample));
+        const uint32_t nActive = (uint32_t)__popcll(__ballot(true));
+        if (nWant < min(TAIL_MIN_LANES, nActive)) continue;
         while (ev != EV_NONE || needSample) {
             bool doScatter = false;  // sample a new direction along which to continue
             uint32_t seg = 0;        // start a segment: 1 along L.dir, 2 toward the sun
