@@ -417,9 +417,16 @@ class RenderTechniquePPB(RenderTechnique):
 
     def __init__(self, ctx, initialRadius=0.002):
         super().__init__(ctx)
+        self.initialRadius = initialRadius
         self.radius = initialRadius
         self.trace_params = None
         self.max_beams = 1 << 22
+
+    def UpdateRadius(self, frameNumber):
+        """RenderTechniquePPB::UpdateRadius (RenderTechniquePPB.cpp:530-546)."""
+        self.radius = self.ctx.lib.cr_ppb_update_radius(frameNumber, structs.CR_PPB_BEAMS_PER_PASS,
+                                                        structs.CR_PPB_ALPHA, self.initialRadius, self.radius)
+        return self.radius
 
     def AllocateResources(self, trace_params=None, max_beams=1 << 22):
         self.trace_params = trace_params
@@ -429,10 +436,13 @@ class RenderTechniquePPB(RenderTechnique):
         pass
 
     def RecordDrawCommands(self, frames, imageIndex=0, gather_ext=None, retrace=True):
+        """One frame per call, as the reference: the radius of frames[0]'s
+        frameCount (RenderTechniquePPB.cpp:439), beams, LBVH, gather."""
         self.UpdateDescriptorSets()
-        if retrace and self.trace_params is not None:
-            self.ctx.trace_beams(self.trace_params, self.max_beams)
-        self.ctx.lbvh_build()
         if isinstance(frames, structs.PushConstants):
             frames = [frames]
+        if retrace and self.trace_params is not None:
+            self.trace_params.radius = self.UpdateRadius(frames[0].frameCount)
+            self.ctx.trace_beams(self.trace_params, self.max_beams)
+        self.ctx.lbvh_build()
         self.ctx.gather(list(frames), gather_ext)

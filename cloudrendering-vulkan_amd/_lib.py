@@ -76,6 +76,7 @@ SIGNATURES = {
     "cr_batch_stats_reset": (st, [ctx_p]),
     "cr_batch_autotune": (st, [ctx_p, C.c_int, P(S.BatchTuning), P(S.BatchTuneReport)]),
     "cr_beam_trace_params_default": (None, [P(S.BeamTraceParams)]),
+    "cr_ppb_update_radius": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
     "cr_ppb_trace_beams": (st, [ctx_p, P(S.BeamTraceParams), C.c_uint32, P(C.c_uint32), P(C.c_uint32)]),
     "cr_beams_upload": (st, [ctx_p, vp, C.c_uint32, vp, C.c_uint32]),
     "cr_beams_count": (st, [ctx_p, P(C.c_uint32), P(C.c_uint32)]),

@@ -216,6 +216,21 @@ void cr_beam_trace_params_default(cr_beam_trace_params* p)
     p->maxInteractions = 64;
 }
 
+float cr_ppb_update_radius(uint32_t frame_number, uint32_t beams_per_pass, float alpha, float initial_radius,
+                           float previous_radius)
+{
+    if (frame_number <= 1u) return initial_radius;
+    // (frameNumber - 2) * m_beamsPerPass is formed in unsigned arithmetic and
+    // stored in an int; constant + j converts it back to unsigned.
+    const int32_t constant = (int32_t)((frame_number - 2u) * beams_per_pass);
+    float sum = 0.0f;
+    for (uint32_t j = 1; j <= beams_per_pass; j++) {
+        const float cj = (float)((uint32_t)constant + j);
+        sum = sum + (cj + alpha) / (cj + 1.0f);
+    }
+    return previous_radius * sum;
+}
+
 void cr_gather_ext_default(cr_gather_ext* g)
 {
     if (!g) return;

@@ -288,6 +288,21 @@ typedef struct cr_beam_trace_params {
 } cr_beam_trace_params;
 CR_API void cr_beam_trace_params_default(cr_beam_trace_params* p);
 
+/* RenderTechniquePPB::UpdateRadius (RenderTechniquePPB.cpp:530-546), called
+ * with PushConstants.frameCount before each frame's beams are traced
+ * (:439): frame_number <= 1 gives initial_radius; otherwise the previous
+ * frame's radius times sum_{j=1..beams_per_pass} (c + j + alpha) / (c + j + 1)
+ * with c = (frame_number - 2) * beams_per_pass, in the reference's integer
+ * (unsigned, then int) and float arithmetic.  The reference's constants are
+ * CR_PPB_ALPHA and CR_PPB_BEAMS_PER_PASS (RenderTechniquePPB.h:115-120).
+ * As written the radius is multiplied by a sum of ratios near 1, so it grows
+ * about beams_per_pass-fold per frame; the function keeps that behaviour (a
+ * caller wanting a fixed radius passes its own to cr_ppb_trace_beams). */
+#define CR_PPB_ALPHA 0.8f
+#define CR_PPB_BEAMS_PER_PASS 56u
+CR_API float cr_ppb_update_radius(uint32_t frame_number, uint32_t beams_per_pass, float alpha, float initial_radius,
+                                  float previous_radius);
+
 /* Trace sun photon paths through the current grid and deposit beams, in path
  * order (deterministic: a counting pass, a scan and a writing pass replace
  * the reference's atomic appends, PPB_PT.comp:340, :381).  Keeps at most

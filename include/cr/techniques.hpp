@@ -152,7 +152,7 @@ public:
 class RenderTechniquePPB : public RenderTechnique {
 public:
     RenderTechniquePPB(Device* d, cr_push_constants* pc, const cr_camera_properties* cam, float initialRadius)
-        : RenderTechnique(d, pc), radius_(initialRadius)
+        : RenderTechnique(d, pc), initialRadius_(initialRadius), radius_(initialRadius)
     {
         if (cam) QueueUpdateCameraProperties(*cam);
         cr_beam_trace_params_default(&trace_);
@@ -163,11 +163,19 @@ public:
     void FreeResources() {}
     void SetTraceParams(const cr_beam_trace_params& p) { trace_ = p; }
     void SetGatherExtensions(const cr_gather_ext& g) { gather_ = g; }
+    // RenderTechniquePPB.cpp:530-546; RecordDrawCommands calls it with the frame's frameCount (:439).
+    void UpdateRadius(unsigned int frameNumber)
+    {
+        radius_ = cr_ppb_update_radius(frameNumber, CR_PPB_BEAMS_PER_PASS, CR_PPB_ALPHA, initialRadius_, radius_);
+    }
+    float Radius() const { return radius_; }
     void RecordDrawCommands(unsigned int = 0) override
     {
         UpdateDescriptorSets();
         cr_context* c = dev_->get();
         uint32_t nb = 0, nd = 0;
+        UpdateRadius(push_->frameCount);
+        trace_.radius = radius_;
         trace_.seed = push_->seed;
         check(c, cr_ppb_trace_beams(c, &trace_, maxBeams_, &nb, &nd), "cr_ppb_trace_beams");
         check(c, cr_lbvh_build(c), "cr_lbvh_build");
@@ -175,6 +183,7 @@ public:
     }
 
 private:
+    float initialRadius_;
     float radius_;
     uint32_t maxBeams_ = 1u << 22;
     cr_beam_trace_params trace_{};

@@ -104,6 +104,10 @@ float or_grid_max(const float* grid, uint64_t n);
 /* Brick majorants: max over each brick^3 brick plus a one-voxel apron. */
 void or_brick_majorants(const float* grid, const uint32_t dims[3], uint32_t brick, float* out, uint32_t mdims_out[3]);
 
+/* progressive beam radius (RenderTechniquePPB.cpp:530-546) */
+float or_ppb_update_radius(uint32_t frameNumber, uint32_t beamsPerPass, float alpha, float initialRadius,
+                           float pmRadius);
+
 #ifdef __cplusplus
 }
 #endif

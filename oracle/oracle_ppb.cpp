@@ -397,6 +397,19 @@ void or_gather_pixel_brute(const or_beam_scene* S, int32_t x, int32_t y, int32_t
     out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
 }
 
+/* RenderTechniquePPB::UpdateRadius, RenderTechniquePPB.cpp:530-546, with the
+ * C++ conversions the reference's declarations imply: the product of two
+ * unsigned ints stored in an int, int + unsigned (unsigned), unsigned + float. */
+float or_ppb_update_radius(uint32_t frameNumber, uint32_t beamsPerPass, float alpha, float initialRadius,
+                           float pmRadius)
+{
+    if (frameNumber <= 1) return initialRadius;
+    float sum = 0;
+    int constant = (int)((frameNumber - 2) * beamsPerPass);
+    for (unsigned int j = 1; j <= beamsPerPass; j++) sum += (constant + j + alpha) / (constant + j + 1.0f);
+    return pmRadius * sum;
+}
+
 void or_gather_render(const or_beam_scene* S, const cr_push_constants* frames, uint32_t n_frames, float* accum,
                       int32_t threads)
 {

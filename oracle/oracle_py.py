@@ -89,6 +89,7 @@ class Oracle:
                                        P(C.c_uint64), P(C.c_uint64)]),
             "or_gather_pixel_brute": (None, [P(OrBeamScene), C.c_int32, C.c_int32, C.c_int32, P(C.c_float)]),
             "or_gather_render": (None, [P(OrBeamScene), P(S.PushConstants), C.c_uint32, vp, C.c_int32]),
+            "or_ppb_update_radius": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -194,6 +195,16 @@ class Oracle:
     def f32_to_f16(self, a):
         a = np.asarray(a, np.float32).ravel()
         return np.array([self.lib.or_f32_to_f16(float(v)) for v in a], np.uint16)
+
+    def ppb_radii(self, frames, beams_per_pass, alpha, initial):
+        """Radius per frame 0..frames-1, each update fed the previous radius
+        (RecordDrawCommands calls UpdateRadius once per frame)."""
+        out = np.empty(frames, np.float32)
+        r = np.float32(0.0)
+        for f in range(frames):
+            r = np.float32(self.lib.or_ppb_update_radius(f, beams_per_pass, alpha, initial, float(r)))
+            out[f] = r
+        return out
 
     # ---------------------------------------------------------------- LBVH
     def lbvh_build(self, beams, b0, b1):

@@ -125,6 +125,19 @@ def test_msvc_seed_sequence(pkg):
     assert list(out) == [38, 7719, 21238, 2437, 8855, 11797, 8365, 32285]
 
 
+def test_ppb_update_radius_matches_oracle(pkg, O, S):
+    """The library's UpdateRadius (a host function) equals the oracle bit for
+    bit over a progressive run, for the reference's constants and others."""
+    lib = pkg.load()
+    for B, alpha, r0 in ((S.CR_PPB_BEAMS_PER_PASS, S.CR_PPB_ALPHA, 0.002), (1, 0.5, 0.01), (300, 0.9, 1.0)):
+        a = b = 0.0
+        for f in range(0, 400):
+            a = lib.cr_ppb_update_radius(f, B, alpha, r0, a)
+            b = O.lib.or_ppb_update_radius(f, B, alpha, r0, b)
+            assert np.float32(a).tobytes() == np.float32(b).tobytes(), (B, f)
+    assert np.array_equal(O.ppb_radii(5, 56, 0.8, 0.002)[:2], np.float32([0.002, 0.002]))
+
+
 def test_configs_are_the_named_workloads(cfg, S):
     c1 = cfg.c1()
     assert (c1.width, c1.height, c1.spp) == (256, 256, 64)

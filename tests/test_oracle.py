@@ -463,3 +463,29 @@ def test_gather_bvh_equals_brute_force_oracle(O, S, pkg, cfg):
             np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-7)
             nonzero += a[0] > 0
     assert nonzero > 10
+
+
+def test_ppb_update_radius_pinned(O):
+    """UpdateRadius (RenderTechniquePPB.cpp:530-546) against what its formula
+    fixes: restart at frames 0 and 1, a sum of exactly B ones for alpha = 1,
+    the single-beam ratio (n - 1 + alpha) / n, the 56-term sum in float64,
+    and ratios that round to 1 once c + j exceeds 2^24."""
+    upd = O.lib.or_ppb_update_radius
+    f32 = np.float32
+    for f in (0, 1):
+        assert upd(f, 56, 0.8, 0.002, 5.0) == f32(0.002)
+    for f in (2, 3, 1000):
+        for B in (1, 7, 56):
+            assert upd(f, B, 1.0, 0.0, 0.25) == 0.25 * B
+    prev = f32(0.01)
+    for f in range(2, 3000):
+        r = upd(f, 1, 0.8, 0.0, float(prev))
+        exact = float(prev) * (f - 1 + float(f32(0.8))) / f
+        assert abs(r - exact) <= 3e-7 * exact, f
+        prev = f32(r)
+    for f, B in ((2, 56), (3, 56), (40, 56), (9, 1000)):
+        c = (f - 2) * B
+        exact = 0.5 * sum((c + j + float(f32(0.8))) / (c + j + 1.0) for j in range(1, B + 1))
+        assert abs(upd(f, B, 0.8, 0.0, 0.5) - exact) <= 4e-7 * B * exact, (f, B)
+    # c = (2^27 - 2) * 56 wraps in 32 bits to 3221225360; past 2^24 each ratio rounds to 1
+    assert upd(1 << 27, 56, 0.8, 0.0, 0.5) == 0.5 * 56
