@@ -39,6 +39,9 @@ constexpr uint32_t COLLISION_CEILING = 1u << 20;  // guards a non-terminating wa
 #define PT_DDA_STEPS_PER_PASS 6
 #endif
 constexpr uint32_t DDA_STEPS_PER_PASS = PT_DDA_STEPS_PER_PASS;  // brick-walk steps per tracking call
+// Skip-grid entries the walk fetches together (the current brick and the next
+// ones along the ray), so their load latencies overlap.
+constexpr int DDA_PREFETCH = 2;
 #ifndef PT_TAIL_MIN_LANES
 #define PT_TAIL_MIN_LANES 16
 #endif
@@ -149,32 +152,34 @@ __device__ __forceinline__ void dda_init(const cr::PTArgs& a, f3 pos, f3 dir, f3
 // (ties: x before y before z); false when the ray leaves the brick grid.
 // Written with selects rather than branches: every lane of a wave takes the
 // same instructions whichever axis it steps.
-__device__ __forceinline__ bool dda_advance(const cr::PTArgs& a, Lane& L)
+__device__ __forceinline__ bool dda_step(const cr::PTArgs& a, int32_t (&cell)[3], float (&tNext)[3], uint32_t& brick)
 {
-    const bool sx = L.tNext[0] <= L.tNext[1] && L.tNext[0] <= L.tNext[2];
-    const bool sy = !sx && L.tNext[1] <= L.tNext[2];
+    const bool sx = tNext[0] <= tNext[1] && tNext[0] <= tNext[2];
+    const bool sy = !sx && tNext[1] <= tNext[2];
     const bool sz = !sx && !sy;
     const DdaAxis d = dda_lds()[threadIdx.x * 3 + (sx ? 0 : (sy ? 1 : 2))];
-    const float tn = (sx ? L.tNext[0] : (sy ? L.tNext[1] : L.tNext[2])) + d.td;
+    const float tn = (sx ? tNext[0] : (sy ? tNext[1] : tNext[2])) + d.td;
     const int32_t step = min(max(d.brickStep, -1), 1);
-    L.cell[0] += sx ? step : 0;
-    L.cell[1] += sy ? step : 0;
-    L.cell[2] += sz ? step : 0;
-    L.tNext[0] = sx ? tn : L.tNext[0];
-    L.tNext[1] = sy ? tn : L.tNext[1];
-    L.tNext[2] = sz ? tn : L.tNext[2];
-    L.brick += (uint32_t)d.brickStep;
-    return ((uint32_t)L.cell[0] < (uint32_t)a.mx) & ((uint32_t)L.cell[1] < (uint32_t)a.my) &
-           ((uint32_t)L.cell[2] < (uint32_t)a.mz);
+    cell[0] += sx ? step : 0;
+    cell[1] += sy ? step : 0;
+    cell[2] += sz ? step : 0;
+    tNext[0] = sx ? tn : tNext[0];
+    tNext[1] = sy ? tn : tNext[1];
+    tNext[2] = sz ? tn : tNext[2];
+    brick += (uint32_t)d.brickStep;
+    return ((uint32_t)cell[0] < (uint32_t)a.mx) & ((uint32_t)cell[1] < (uint32_t)a.my) &
+           ((uint32_t)cell[2] < (uint32_t)a.mz);
 }
+
+__device__ __forceinline__ bool dda_advance(const cr::PTArgs& a, Lane& L) { return dda_step(a, L.cell, L.tNext, L.brick); }
 
 // The skip grid, two floats per brick: (majorant * densityScaling) /
 // baseScaling of a non-empty brick, -d for an empty brick d bricks from the
 // nearest non-empty one; then the brick's majorant.  Brick tables hold < 2^29
 // entries, so the byte offset fits 32 bits.
-__device__ __forceinline__ float2 brick_skip(const cr::PTArgs& a, const Lane& L)
+__device__ __forceinline__ float2 brick_skip(const cr::PTArgs& a, uint32_t brick)
 {
-    return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(a.skip) + (L.brick << 3));
+    return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(a.skip) + (brick << 3));
 }
 
 // Start a tracking segment from L.pos along `dir` (primary or shadow).
@@ -207,52 +212,52 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
     const f3 dir = shadow ? sdir : L.dir;
     if (TRACK == CR_TRACK_BRICK_DDA) {
         float maj = 0.0f;
-        // Empty-space skipping: the skip grid holds -d for an empty brick whose
-        // nearest non-empty brick is d bricks away (Chebyshev), so every brick
-        // of the cube of radius d-1 around it is empty; while the walk stays in
-        // that cube no majorant is fetched.  The arithmetic of the walk is
-        // unchanged (an empty brick has majorant 0 either way).
-        // The cube is kept as its low corner and edge span: cell - low, taken
-        // unsigned, is <= span exactly inside it.  The initial corner 2^30 lies
-        // beyond every brick, so no brick is inside before the first fetch.
-        int32_t lx = 1 << 30, ly = 1 << 30, lz = 1 << 30;
-        uint32_t span = 0;
+        // Empty bricks (skip entry <= 0) are crossed with no lookup.  Every
+        // brick's entry is fetched, two at a time: with the loads overlapped,
+        // that is cheaper than testing whether the walk is still inside a cube
+        // of bricks the skip grid's distances prove empty.
         // At most DDA_STEPS_PER_PASS bricks per call: a lane crossing a long
         // stretch of thin or empty bricks resumes in the next pass instead of
         // holding its wave's other lanes idle (the walk's state is in Lane).
-        for (uint32_t it = 0;; it++) {
-            if (it == DDA_STEPS_PER_PASS) return EV_NONE;
-            // sb = (majorant * densityScaling) / baseScaling, precomputed per
-            // brick in the skip grid (the same IEEE division as here)
-            float sb;
-            const bool inside = ((uint32_t)(L.cell[0] - lx) <= span) & ((uint32_t)(L.cell[1] - ly) <= span) &
-                                ((uint32_t)(L.cell[2] - lz) <= span);
-            if (inside) {
-                sb = 0.0f;
-            } else {
-                const float2 sk = brick_skip(a, L);
-                sb = sk.x;
-                maj = sk.y;
-                if (sb < 0.0f) {
-                    const int32_t r = (int32_t)(-sb) - 1;
-                    lx = L.cell[0] - r;
-                    ly = L.cell[1] - r;
-                    lz = L.cell[2] - r;
-                    span = (uint32_t)(2 * r);
-                    sb = 0.0f;
+        bool collided = false;
+        for (uint32_t it = 0; !collided; it += DDA_PREFETCH) {
+            if (it >= DDA_STEPS_PER_PASS) return EV_NONE;
+            // The entries of this brick and the next DDA_PREFETCH - 1 along the
+            // ray, loaded back to back (a copy of the walk's state steps ahead;
+            // past the grid's edge brick 0 is loaded and never used).
+            float2 pre[DDA_PREFETCH];
+            {
+                int32_t cell[3] = {L.cell[0], L.cell[1], L.cell[2]};
+                float tNext[3] = {L.tNext[0], L.tNext[1], L.tNext[2]};
+                uint32_t brick = L.brick;
+                bool in = true;
+                pre[0] = brick_skip(a, brick);
+#pragma unroll
+                for (int k = 1; k < DDA_PREFETCH; k++) {
+                    in = in & dda_step(a, cell, tNext, brick);
+                    pre[k] = brick_skip(a, in ? brick : 0u);
                 }
             }
-            float tc = fminf(fminf(fminf(L.tNext[0], L.tNext[1]), L.tNext[2]), L.dist);
-            float seg = tc - L.t;
-            c.brickVisits++;
-            if (sb > 0.0f && L.tau < sb * seg) {
-                L.t = L.t + fdiv(L.tau, sb);
-                break;
+#pragma unroll
+            for (int k = 0; k < DDA_PREFETCH; k++) {
+                // sb = (majorant * densityScaling) / baseScaling, precomputed per
+                // brick in the skip grid (the same IEEE division as here); 0 for
+                // an empty brick
+                const float sb = fmaxf(pre[k].x, 0.0f);
+                maj = pre[k].y;
+                float tc = fminf(fminf(fminf(L.tNext[0], L.tNext[1]), L.tNext[2]), L.dist);
+                float seg = tc - L.t;
+                c.brickVisits++;
+                if (sb > 0.0f && L.tau < sb * seg) {
+                    L.t = L.t + fdiv(L.tau, sb);
+                    collided = true;
+                    break;
+                }
+                L.tau = L.tau - sb * seg;
+                L.t = tc;
+                if (!(tc < L.dist)) return EV_EXIT;
+                if (!dda_advance(a, L)) return EV_EXIT;
             }
-            L.tau = L.tau - sb * seg;
-            L.t = tc;
-            if (!(tc < L.dist)) return EV_EXIT;
-            if (!dda_advance(a, L)) return EV_EXIT;
         }
         // the loop leaves only from a brick whose entry it fetched (sb > 0)
         f3 cur = add(L.pos, mul(dir, L.t));
