@@ -37,13 +37,28 @@ __device__ __forceinline__ f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
 __device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 // Correctly rounded binary32 square root.  The hardware square root
 // (v_sqrt_f32, which __fsqrt_rn lowers to) is accurate to one ULP only, so its
-// result s is corrected against the midpoints between s and its two
-// neighbours: a midpoint has 25 significant bits and its square 50, so the
-// products and comparisons below are exact in binary64, and no binary32 input
-// can equal such a square (no ties).
+// result s is corrected.
+//  * a in [2^-100, FLT_MAX]: with s- and s+ the neighbours of s, the residuals
+//    a - s- s and a - s+ s are multiples of ulp(s)^2, as a is, and the squared
+//    midpoints lie a quarter of ulp(s)^2 above those products; so the signs of
+//    the two FMA residuals (exact: no underflow in this range) decide whether
+//    the square root lies below the lower or above the upper midpoint.
+//    Checked exhaustively on the CPU for every such a and every start value
+//    within one step of the correctly rounded result.
+//  * other inputs: s is compared with the midpoints between s and its two
+//    neighbours in binary64, where a midpoint (25 significant bits) and its
+//    square (50) are exact, and no binary32 input equals such a square.
 __device__ __forceinline__ float fsqrt(float a)
 {
     const float s = __fsqrt_rn(a);
+    if (a >= 0x1p-100f && a <= 3.40282347e38f) {
+        const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+        const float su = __uint_as_float(__float_as_uint(s) + 1u);
+        const float vd = __builtin_fmaf(-sd, s, a);
+        const float vu = __builtin_fmaf(-su, s, a);
+        const float r = vd <= 0.0f ? sd : s;
+        return vu > 0.0f ? su : r;
+    }
     if (!(a > 0.0f) || a == __builtin_inff()) return s;
     const double ad = a;
     const float lo = __uint_as_float(__float_as_uint(s) - 1u);
@@ -120,13 +135,12 @@ __device__ __forceinline__ float crd_expf(float x)
     return p * __uint_as_float((uint32_t)(ki + 127) << 23);
 }
 
-__device__ __forceinline__ void crd_sincosf(float x, float* s_out, float* c_out)
+// Written with selects rather than branches (the quadrant and the domain), so
+// the lanes of a wave run it together.
+__device__ __forceinline__ void crd_sincosf(float x_in, float* s_out, float* c_out)
 {
-    if (!(fabsf(x) <= 131072.0f)) {
-        *s_out = __builtin_nanf("");
-        *c_out = __builtin_nanf("");
-        return;
-    }
+    const bool inDomain = fabsf(x_in) <= 131072.0f;  // false for NaN
+    const float x = inDomain ? x_in : 0.0f;
     float k = floorf(x * 0.636619772f + 0.5f);
     float r = ((x - k * 1.5703125f) - k * 4.83751296997070312e-4f) - k * 7.54978995489188216e-8f;
     float z = r * r;
@@ -143,14 +157,14 @@ __device__ __forceinline__ void crd_sincosf(float x, float* s_out, float* c_out)
     pc = pc * z + 4.16666667e-2f;
     pc = pc * z - 0.5f;
     float cs = 1.0f + z * pc;
-    int32_t q = ((int32_t)k) & 3;
-    float so, co;
-    if (q == 0) { so = sn; co = cs; }
-    else if (q == 1) { so = cs; co = -sn; }
-    else if (q == 2) { so = -sn; co = -cs; }
-    else { so = -cs; co = sn; }
-    *s_out = so;
-    *c_out = co;
+    // quadrant q: (sin, cos) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn)
+    const int32_t q = ((int32_t)k) & 3;
+    const float sa = (q & 1) ? cs : sn;
+    const float ca = (q & 1) ? sn : cs;
+    const float so = (q & 2) ? -sa : sa;
+    const float co = ((q + 1) & 2) ? -ca : ca;
+    *s_out = inDomain ? so : __builtin_nanf("");
+    *c_out = inDomain ? co : __builtin_nanf("");
 }
 
 __device__ __forceinline__ float pow15(float x) { return x * fsqrt(x); }
