@@ -41,15 +41,15 @@ def parse():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5"],
                     help="c1/c2: plain path tracer throughput; c4: 64 jobs batched vs one by one; "
                          "c5: learned vs plain time to relMSE <= 1e-3")
-    ap.add_argument("--c5-clouds", type=int, default=200)
+    ap.add_argument("--c5-clouds", type=int, default=400)
     ap.add_argument("--c5-points", type=int, default=80000, help="training points per cloud")
     ap.add_argument("--c5-target-spp", type=int, default=1024, help="continuations per training point")
-    ap.add_argument("--c5-train-steps", type=int, default=40000)
+    ap.add_argument("--c5-train-steps", type=int, default=80000)
     ap.add_argument("--c5-collect-frames", type=int, default=8, help="record-pass frames per training cloud")
     ap.add_argument("--c5-tracking", default="dda", choices=["dda", "global"],
                     help="dda: brick-majorant tracking + ratio-tracked shadow rays (the production tracker); "
                          "global: the reference's global majorant + shadow-volume NEE (c1's settings)")
-    ap.add_argument("--c5-batch", type=int, default=4096)
+    ap.add_argument("--c5-batch", type=int, default=8192)
     ap.add_argument("--c5-ref-spp", type=int, default=16384, help="frames of the converged plain reference")
     ap.add_argument("--c5-width", type=int, default=1920)
     ap.add_argument("--c5-height", type=int, default=1080)
