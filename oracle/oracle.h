@@ -104,6 +104,9 @@ float or_grid_max(const float* grid, uint64_t n);
 /* Brick majorants: max over each brick^3 brick plus a one-voxel apron. */
 void or_brick_majorants(const float* grid, const uint32_t dims[3], uint32_t brick, float* out, uint32_t mdims_out[3]);
 
+/* rounding correction of the kernels' square root, checked over [lo_bits, hi_bits] */
+uint64_t or_sqrt_correction_errors(uint32_t lo_bits, uint32_t hi_bits);
+
 /* progressive beam radius (RenderTechniquePPB.cpp:530-546) */
 float or_ppb_update_radius(uint32_t frameNumber, uint32_t beamsPerPass, float alpha, float initialRadius,
                            float pmRadius);

@@ -90,6 +90,7 @@ class Oracle:
             "or_gather_pixel_brute": (None, [P(OrBeamScene), C.c_int32, C.c_int32, C.c_int32, P(C.c_float)]),
             "or_gather_render": (None, [P(OrBeamScene), P(S.PushConstants), C.c_uint32, vp, C.c_int32]),
             "or_ppb_update_radius": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
+            "or_sqrt_correction_errors": (C.c_uint64, [C.c_uint32, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

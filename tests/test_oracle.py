@@ -489,3 +489,14 @@ def test_ppb_update_radius_pinned(O):
         assert abs(upd(f, B, 0.8, 0.0, 0.5) - exact) <= 4e-7 * B * exact, (f, B)
     # c = (2^27 - 2) * 56 wraps in 32 bits to 3221225360; past 2^24 each ratio rounds to 1
     assert upd(1 << 27, 56, 0.8, 0.0, 0.5) == 0.5 * 56
+
+
+def test_sqrt_rounding_correction_exhaustive(O):
+    """The FMA-residual correction the kernels apply to the hardware square
+    root returns the correctly rounded root for every input in [2^-100,
+    FLT_MAX] from each of the three start values within one step of it (all
+    2^31-odd inputs; a few seconds with OpenMP).  The device side of the same
+    claim is test_gpu_math.py's comparison with IEEE sqrt."""
+    lo = int(np.float32(2.0 ** -100).view(np.uint32))
+    hi = 0x7F7FFFFF
+    assert O.lib.or_sqrt_correction_errors(lo, hi) == 0
