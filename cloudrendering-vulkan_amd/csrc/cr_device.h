@@ -44,7 +44,8 @@ __device__ __forceinline__ float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y
 //    the two FMA residuals (exact: no underflow in this range) decide whether
 //    the square root lies below the lower or above the upper midpoint.
 //    Checked exhaustively on the CPU for every such a and every start value
-//    within one step of the correctly rounded result.
+//    within one step of the correctly rounded result (the oracle's
+//    or_sqrt_correction_errors).
 //  * other inputs: s is compared with the midpoints between s and its two
 //    neighbours in binary64, where a midpoint (25 significant bits) and its
 //    square (50) are exact, and no binary32 input equals such a square.
