@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
                     help="c1/c2: plain path tracer throughput; c4: 64 jobs batched vs one by one; "
                          "c5: learned vs plain time to relMSE <= 1e-3")
     ap.add_argument("--c5-clouds", type=int, default=400)
@@ -80,6 +80,8 @@ def main():
         return bench_c5(args, pkg, cfg)
     if args.workload == "c4":
         return bench_c4(args, pkg, cfg)
+    if args.workload == "c3":
+        return bench_c3(args, pkg, cfg)
 
     sc = cfg.c2() if args.workload == "c2" else cfg.c1()
     ctx = pkg.Context(0)
@@ -265,6 +267,111 @@ def bench_c4(args, pkg, cfg):
         "tuning": {"blockThreads": tun.blockThreads, "blocksPerCU": tun.blocksPerCU, "tileWidth": tun.tileWidth,
                    "candidates": rep.candidates, "autotune_s": round(t_tune, 2)},
         "images_identical_to_single_renders": {"default": same_default, "tuned": same_tuned, "jobs": len(jobs)},
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_c3(args, pkg, cfg):
+    """Photon beams on the c1 volume: 2^22 sun-emitted beams traced once and
+    then frozen, the LBVH build over them (30-bit Morton codes, 4 x 8-bit LSD
+    radix sort, Karras hierarchy, AABB refit) timed over --steps builds after
+    --warmup and checked bit-exact against the oracle at full size, then the
+    1080p beam gather at 16 spp.  value = beams per second through the build."""
+    import numpy as np
+    S = pkg.structs
+    O = None if args.no_cpu else __import__("cr_import").oracle()
+    ctx = pkg.Context(0)
+    sc = cfg.c1(width=1920, height=1080)
+    cloud, _ = sc.setup(ctx, build_sv=False)
+    # camera transmittance for the gather: a shadow volume along the camera
+    # forward axis (main.cpp:226-229)
+    ctx.set_shadow_volume(cfg.shadow_volume(cloud, tuple(sc.cam.forward), 256))
+    ctx.sv_build()
+    side = max(cloud.bounds[1][k] - cloud.bounds[0][k] for k in range(3))
+    target = 1 << 22
+    paths = 1 << 19
+    t0 = time.perf_counter()
+    while True:
+        p = cfg.beam_params(paths, seed=38, radius=0.002 * side, albedo=0.9, light_dir=sc.light_dir,
+                            max_interactions=64)
+        nb, nd = ctx.trace_beams(p, target)
+        ctx.synchronize()
+        if nb >= target or paths >= (1 << 24):
+            break
+        paths *= 2
+    t_trace = time.perf_counter() - t0
+    log(f"c3: {nb} beams ({nd} beam data) from {paths} photon paths traced in {t_trace:.2f}s")
+    stage_ms, wall = [], []
+    for k in range(args.warmup + args.steps):
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.lbvh_build()
+        ctx.synchronize()
+        if k >= args.warmup:
+            wall.append(time.perf_counter() - t)
+            stage_ms.append(ctx.lbvh_timings())
+    build_ms = float(np.mean([sum(v for key, v in st.items() if key != "pack") for st in stage_ms]))
+    stages = {key: round(float(np.mean([st[key] for st in stage_ms])), 4) for key in stage_ms[0]}
+    keys, perm, nodes = ctx.lbvh_download()
+    parity = None
+    cpu = None
+    if O is not None:
+        beams, _ = ctx.download_beams()
+        t = time.perf_counter()
+        rk, rp, rn = O.lbvh_build(beams, cloud.bounds[0], cloud.bounds[1])
+        t_cpu = time.perf_counter() - t
+        same_nodes = bytes(nodes)[: (2 * nb - 1) * 64] == bytes(rn)[: (2 * nb - 1) * 64]
+        parity = {"keys": bool(np.array_equal(keys, rk)), "perm": bool(np.array_equal(perm, rp)),
+                  "nodes": bool(same_nodes)}
+        cpu = {"value": round(nb / t_cpu / 1e6, 3), "unit": "Mbeams/s (LBVH build)", "cores": 1, "kind": "port",
+               "sample": f"oracle (oracle/liboracle.so, C++ -O3) building the LBVH of the same {nb} beams "
+                         f"(Morton codes, radix sort, Karras hierarchy, refit): {t_cpu:.2f}s"}
+        log(f"c3: oracle build {t_cpu:.2f}s; parity {parity}")
+    # gather: 1080p x 16 spp in one call
+    frames = cfg.frames(16)
+    ext = S.GatherExt(0, 0, 0.9, 0)
+    ctx.clear_image()
+    ctx.gather(frames[:1], ext)      # warm-up frame
+    ctx.synchronize()
+    ctx.clear_image()
+    g0 = ctx.gather_stats()
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.gather(frames, ext)
+    ctx.synchronize()
+    t_gather = time.perf_counter() - t
+    g1 = ctx.gather_stats()
+    rays, nodes_v, tested = g1.rays - g0.rays, g1.nodesVisited - g0.nodesVisited, g1.beamsTested - g0.beamsTested
+    img = ctx.download_image(sc.height, sc.width)
+    # algorithmic bytes of the build per beam: Morton 24 B read + 8 B written, sort 4 x (4 + 8 + 8) B,
+    # hierarchy ~20 B, refit 28 B read + 48 B written
+    alg = 208.0 * nb
+    achieved = alg / (build_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(nb / (build_ms * 1e-3) / 1e6, 2),
+        "unit": "Mbeams/s (LBVH build: Morton, sort, hierarchy, refit; HIP events)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32 beams, u32 keys",
+        "data": "synthetic",
+        "config": {"workload": f"c3: {nb} sun-emitted beams traced on the c1 volume (seed 38, radius "
+                               f"0.002 x box side), LBVH build, 1920x1080 beam gather at 16 spp"},
+        "build_ms": round(build_ms, 4),
+        "build_wall_ms": round(1e3 * float(np.mean(wall)), 3),
+        "stages_ms": stages,
+        "trace_s": round(t_trace, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": "lbvh build (5 stages)",
+                     "algorithmic_bytes_per_build": alg},
+        "parity_vs_oracle": parity,
+        "cpu_baseline": cpu,
+        "gather": {"spp": 16, "seconds": round(t_gather, 4), "kernel_ms": round(g1.lastKernelMs, 3),
+                   "Mrays_per_s": round(rays / (g1.lastKernelMs * 1e-3) / 1e6, 2),
+                   "nodes_per_ray": round(nodes_v / max(rays, 1), 2),
+                   "beams_tested_per_ray": round(tested / max(rays, 1), 2),
+                   "finite": bool(np.isfinite(img).all())},
     }
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -15,9 +15,11 @@ struct BoxArg {
 };
 
 // One internal node in traversal form: both child boxes and indices.
+// Center and half extent of each child's box, as intersectBox
+// (PPB_PE.comp:282-300) forms them from the bounds, computed once at build.
 struct TravNode {
-    float lo[2][3];
-    float hi[2][3];
+    float cen[2][3];
+    float rad[2][3];
     uint32_t child[2];
     uint32_t leafMask;  // bit c set: child c is a leaf
     uint32_t pad;

@@ -261,10 +261,13 @@ __global__ void __launch_bounds__(256) pack_kernel(uint32_t n, const uint32_t* l
     cr::TravNode t;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        t.lo[0][k] = lo[3 * (size_t)a + k];
-        t.hi[0][k] = hi[3 * (size_t)a + k];
-        t.lo[1][k] = lo[3 * (size_t)b + k];
-        t.hi[1][k] = hi[3 * (size_t)b + k];
+        const float la = lo[3 * (size_t)a + k], ha = hi[3 * (size_t)a + k];
+        const float lb = lo[3 * (size_t)b + k], hb = hi[3 * (size_t)b + k];
+        // (lo + hi) / 2 and (hi - lo) * 0.5: halving is exact, so * 0.5 is the same value
+        t.cen[0][k] = (la + ha) * 0.5f;
+        t.rad[0][k] = (ha - la) * 0.5f;
+        t.cen[1][k] = (lb + hb) * 0.5f;
+        t.rad[1][k] = (hb - lb) * 0.5f;
     }
     t.child[0] = a;
     t.child[1] = b;

@@ -182,12 +182,12 @@ __global__ void __launch_bounds__(256) trace_kernel(cr::TraceArgs a)
     }
 }
 
-// intersectBox (PPB_PE.comp:282-300) with rad = half the box size.
-__device__ __forceinline__ bool hit_box(f3 ro, f3 rd, const float* lo, const float* hi)
+// intersectBox (PPB_PE.comp:282-300) with rad = half the box size; the box as
+// center and half extent, m = 1 / rd formed once per ray.
+__device__ __forceinline__ bool hit_box(f3 ro, f3 m, const float* cen3, const float* rad3)
 {
-    const f3 cen = mk(fdiv(lo[0] + hi[0], 2.0f), fdiv(lo[1] + hi[1], 2.0f), fdiv(lo[2] + hi[2], 2.0f));
-    const f3 rad = mk((hi[0] - lo[0]) * 0.5f, (hi[1] - lo[1]) * 0.5f, (hi[2] - lo[2]) * 0.5f);
-    const f3 m = mk(fdiv(1.0f, rd.x), fdiv(1.0f, rd.y), fdiv(1.0f, rd.z));
+    const f3 cen = mk(cen3[0], cen3[1], cen3[2]);
+    const f3 rad = mk(rad3[0], rad3[1], rad3[2]);
     const f3 n = mk(m.x * (ro.x - cen.x), m.y * (ro.y - cen.y), m.z * (ro.z - cen.z));
     const f3 k = mk(fabsf(m.x) * rad.x, fabsf(m.y) * rad.y, fabsf(m.z) * rad.z);
     const float tN = fmaxf(fmaxf(-n.x - k.x, -n.y - k.y), -n.z - k.z);
@@ -280,8 +280,14 @@ __global__ void __launch_bounds__(GATHER_THREADS) gather_kernel(cr::GatherArgs a
             rays++;
             f4 L;
             L.x = 0.0f; L.y = 0.0f; L.z = 0.0f; L.w = 0.0f;
+            const f3 m = mk(fdiv(1.0f, rd.x), fdiv(1.0f, rd.y), fdiv(1.0f, rd.z));
             if (a.n == 1) {
-                if (hit_box(ro, rd, a.leafLo, a.leafHi)) {
+                // (lo + hi) / 2: halving is exact, so * 0.5 is the same value
+                const float cen[3] = {(a.leafLo[0] + a.leafHi[0]) * 0.5f, (a.leafLo[1] + a.leafHi[1]) * 0.5f,
+                                      (a.leafLo[2] + a.leafHi[2]) * 0.5f};
+                const float rad[3] = {(a.leafHi[0] - a.leafLo[0]) * 0.5f, (a.leafHi[1] - a.leafLo[1]) * 0.5f,
+                                      (a.leafHi[2] - a.leafLo[2]) * 0.5f};
+                if (hit_box(ro, m, cen, rad)) {
                     beam_radiance<T, LAYOUT>(a, 0, ro, rd, L);
                     tested++;
                 }
@@ -300,8 +306,8 @@ __global__ void __launch_bounds__(GATHER_THREADS) gather_kernel(cr::GatherArgs a
                     }
                     visited++;
                     const cr::TravNode nd = a.nodes[cur];
-                    const bool hl = hit_box(ro, rd, nd.lo[0], nd.hi[0]);
-                    const bool hr = hit_box(ro, rd, nd.lo[1], nd.hi[1]);
+                    const bool hl = hit_box(ro, m, nd.cen[0], nd.rad[0]);
+                    const bool hr = hit_box(ro, m, nd.cen[1], nd.rad[1]);
                     const bool ll = (nd.leafMask & 1u) != 0, lr = (nd.leafMask & 2u) != 0;
                     if (hl && ll) { beam_radiance<T, LAYOUT>(a, nd.child[0] - inner, ro, rd, L); tested++; }
                     if (hr && lr) { beam_radiance<T, LAYOUT>(a, nd.child[1] - inner, ro, rd, L); tested++; }
