@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--c5-width", type=int, default=1920)
     ap.add_argument("--c5-height", type=int, default=1080)
     ap.add_argument("--spp-per-step", type=int, default=0, help="frames per step; 0 = the workload's full spp")
+    ap.add_argument("--c3-gather-spp", type=int, default=16, help="frames of the c3 beam gather (0: no gather)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--via-batch", action="store_true",
@@ -328,22 +329,29 @@ def bench_c3(args, pkg, cfg):
                "sample": f"oracle (oracle/liboracle.so, C++ -O3) building the LBVH of the same {nb} beams "
                          f"(Morton codes, radix sort, Karras hierarchy, refit): {t_cpu:.2f}s"}
         log(f"c3: oracle build {t_cpu:.2f}s; parity {parity}")
-    # gather: 1080p x 16 spp in one call
-    frames = cfg.frames(16)
+    # gather: 1080p x --c3-gather-spp frames in one call
+    frames = cfg.frames(max(1, args.c3_gather_spp))
     ext = S.GatherExt(0, 0, 0.9, 0)
     ctx.clear_image()
     ctx.gather(frames[:1], ext)      # warm-up frame
+    gather = None
     ctx.synchronize()
-    ctx.clear_image()
-    g0 = ctx.gather_stats()
-    ctx.synchronize()
-    t = time.perf_counter()
-    ctx.gather(frames, ext)
-    ctx.synchronize()
-    t_gather = time.perf_counter() - t
-    g1 = ctx.gather_stats()
-    rays, nodes_v, tested = g1.rays - g0.rays, g1.nodesVisited - g0.nodesVisited, g1.beamsTested - g0.beamsTested
-    img = ctx.download_image(sc.height, sc.width)
+    if args.c3_gather_spp > 0:
+        ctx.clear_image()
+        g0 = ctx.gather_stats()
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.gather(frames, ext)
+        ctx.synchronize()
+        t_gather = time.perf_counter() - t
+        g1 = ctx.gather_stats()
+        rays, nodes_v, tested = g1.rays - g0.rays, g1.nodesVisited - g0.nodesVisited, g1.beamsTested - g0.beamsTested
+        img = ctx.download_image(sc.height, sc.width)
+        gather = {"spp": args.c3_gather_spp, "seconds": round(t_gather, 4), "kernel_ms": round(g1.lastKernelMs, 3),
+                  "Mrays_per_s": round(rays / (g1.lastKernelMs * 1e-3) / 1e6, 2),
+                  "nodes_per_ray": round(nodes_v / max(rays, 1), 2),
+                  "beams_tested_per_ray": round(tested / max(rays, 1), 2),
+                  "finite": bool(np.isfinite(img).all())}
     # algorithmic bytes of the build per beam: Morton 24 B read + 8 B written, sort 4 x (4 + 8 + 8) B,
     # hierarchy ~20 B, refit 28 B read + 48 B written
     alg = 208.0 * nb
@@ -357,7 +365,8 @@ def bench_c3(args, pkg, cfg):
         "dtype": "f32 beams, u32 keys",
         "data": "synthetic",
         "config": {"workload": f"c3: {nb} sun-emitted beams traced on the c1 volume (seed 38, radius "
-                               f"0.002 x box side), LBVH build, 1920x1080 beam gather at 16 spp"},
+                               f"0.002 x box side), LBVH build, 1920x1080 beam gather at "
+                               f"{args.c3_gather_spp} spp"},
         "build_ms": round(build_ms, 4),
         "build_wall_ms": round(1e3 * float(np.mean(wall)), 3),
         "stages_ms": stages,
@@ -367,11 +376,7 @@ def bench_c3(args, pkg, cfg):
                      "algorithmic_bytes_per_build": alg},
         "parity_vs_oracle": parity,
         "cpu_baseline": cpu,
-        "gather": {"spp": 16, "seconds": round(t_gather, 4), "kernel_ms": round(g1.lastKernelMs, 3),
-                   "Mrays_per_s": round(rays / (g1.lastKernelMs * 1e-3) / 1e6, 2),
-                   "nodes_per_ray": round(nodes_v / max(rays, 1), 2),
-                   "beams_tested_per_ray": round(tested / max(rays, 1), 2),
-                   "finite": bool(np.isfinite(img).all())},
+        "gather": gather,
     }
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -31,12 +31,14 @@ hipError_t launch_morton(const cr_photon_beam* beams, uint32_t n, const BoxArg& 
 uint32_t sort_tiles(uint32_t n);
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* counts,
                              uint32_t n, hipStream_t s);
+// ends[i]: the other end j of internal node i's leaf range [min(i, j), max(i, j)]
 hipError_t launch_hierarchy(const uint32_t* keys, uint32_t n, uint32_t* parent, uint32_t* left, uint32_t* right,
-                            hipStream_t s);
+                            uint32_t* ends, hipStream_t s);
 hipError_t launch_leaves(const cr_photon_beam* beams, const uint32_t* perm, const uint32_t* keys, uint32_t n,
                          cr_photon_beam* sorted, float* lo, float* hi, hipStream_t s);
-hipError_t launch_refit(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, float* lo, float* hi,
-                        uint32_t* arrivals, hipStream_t s);
+// exits: room for 2n node indices; exitCount and arrivals (n - 1 counters) zeroed beforehand
+hipError_t launch_refit(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, const uint32_t* ends,
+                        float* lo, float* hi, uint32_t* arrivals, uint32_t* exits, uint32_t* exitCount, hipStream_t s);
 hipError_t launch_pack(uint32_t n, const uint32_t* left, const uint32_t* right, const float* lo, const float* hi,
                        TravNode* out, hipStream_t s);
 hipError_t launch_treenodes(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, const float* lo,

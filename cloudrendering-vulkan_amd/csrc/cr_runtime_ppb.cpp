@@ -24,8 +24,8 @@ struct Ppb {
     size_t sortedCap = 0;
     uint32_t *keys = nullptr, *vals = nullptr, *keysTmp = nullptr, *valsTmp = nullptr, *counts = nullptr;
     size_t keysCap = 0, valsCap = 0, keysTmpCap = 0, valsTmpCap = 0, countsCap = 0;
-    uint32_t *parent = nullptr, *left = nullptr, *right = nullptr, *arrivals = nullptr;
-    size_t parentCap = 0, leftCap = 0, rightCap = 0, arrivalsCap = 0;
+    uint32_t *parent = nullptr, *left = nullptr, *right = nullptr, *arrivals = nullptr, *ends = nullptr, *exits = nullptr;
+    size_t parentCap = 0, leftCap = 0, rightCap = 0, arrivalsCap = 0, endsCap = 0, exitsCap = 0;
     float *lo = nullptr, *hi = nullptr;
     size_t loCap = 0, hiCap = 0;
     TravNode* trav = nullptr;
@@ -48,7 +48,7 @@ void ppb_free(cr_context* ctx)
     Ppb* p = ctx->ppb;
     if (!p) return;
     void* bufs[] = {p->beams, p->data, p->sorted, p->keys, p->vals, p->keysTmp, p->valsTmp, p->counts, p->parent,
-                    p->left, p->right, p->arrivals, p->lo, p->hi, p->trav, p->gcounters, p->overflow};
+                    p->left, p->right, p->arrivals, p->ends, p->exits, p->lo, p->hi, p->trav, p->gcounters, p->overflow};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : p->ev)
@@ -215,6 +215,9 @@ cr_status cr_lbvh_build(cr_context* ctx)
     CR_TRY(ensure_buffer(ctx, (void**)&p->left, &p->leftCap, (size_t)n * 4, "left"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->right, &p->rightCap, (size_t)n * 4, "right"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->arrivals, &p->arrivalsCap, (size_t)n * 4, "arrivals"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->ends, &p->endsCap, (size_t)n * 4, "range ends"));
+    // refit exits (at most two per internal node) and their count at the end
+    CR_TRY(ensure_buffer(ctx, (void**)&p->exits, &p->exitsCap, (2 * (size_t)n + 1) * 4, "refit exits"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->lo, &p->loCap, nodes * 12, "box lo"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->hi, &p->hiCap, nodes * 12, "box hi"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->trav, &p->travCap, (size_t)n * sizeof(TravNode), "traversal nodes"));
@@ -225,16 +228,18 @@ cr_status cr_lbvh_build(cr_context* ctx)
     }
     hipStream_t s = ctx->stream;
     CR_HIP(ctx, zero_async(p->arrivals, (size_t)n * 4, s), "zero arrivals");
+    CR_HIP(ctx, zero_async(p->exits + 2 * (size_t)n, 4, s), "zero exit count");
     if (n == 1) CR_HIP(ctx, zero_async(p->parent, 4, s), "zero parent");
     CR_HIP(ctx, hipEventRecord(p->ev[0], s), "event");
     CR_HIP(ctx, launch_morton(p->beams, n, box, p->keys, p->vals, s), "morton kernel");
     CR_HIP(ctx, hipEventRecord(p->ev[1], s), "event");
     CR_HIP(ctx, launch_radix_sort(p->keys, p->vals, p->keysTmp, p->valsTmp, p->counts, n, s), "radix sort");
     CR_HIP(ctx, hipEventRecord(p->ev[2], s), "event");
-    CR_HIP(ctx, launch_hierarchy(p->keys, n, p->parent, p->left, p->right, s), "hierarchy kernel");
+    CR_HIP(ctx, launch_hierarchy(p->keys, n, p->parent, p->left, p->right, p->ends, s), "hierarchy kernel");
     CR_HIP(ctx, hipEventRecord(p->ev[3], s), "event");
     CR_HIP(ctx, launch_leaves(p->beams, p->vals, p->keys, n, p->sorted, p->lo, p->hi, s), "leaf kernel");
-    CR_HIP(ctx, launch_refit(n, p->parent, p->left, p->right, p->lo, p->hi, p->arrivals, s), "refit kernel");
+    CR_HIP(ctx, launch_refit(n, p->parent, p->left, p->right, p->ends, p->lo, p->hi, p->arrivals, p->exits,
+                             p->exits + 2 * (size_t)n, s), "refit kernels");
     CR_HIP(ctx, hipEventRecord(p->ev[4], s), "event");
     CR_HIP(ctx, launch_pack(n, p->left, p->right, p->lo, p->hi, p->trav, s), "pack kernel");
     CR_HIP(ctx, hipEventRecord(p->ev[5], s), "event");

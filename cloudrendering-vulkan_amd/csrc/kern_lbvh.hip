@@ -162,7 +162,7 @@ __device__ __forceinline__ int delta(const uint32_t* k, int32_t n, int32_t i, in
 }
 
 __global__ void __launch_bounds__(256) hierarchy_kernel(const uint32_t* k, int32_t n, uint32_t* parent, uint32_t* left,
-                                                        uint32_t* right)
+                                                        uint32_t* right, uint32_t* ends)
 {
     const int32_t i = (int32_t)(blockIdx.x * 256 + threadIdx.x);
     if (i >= n - 1) return;
@@ -188,6 +188,7 @@ __global__ void __launch_bounds__(256) hierarchy_kernel(const uint32_t* k, int32
     const uint32_t rc = (last == s + 1) ? inner + (uint32_t)(s + 1) : (uint32_t)(s + 1);
     left[i] = lc;
     right[i] = rc;
+    ends[i] = (uint32_t)j;  // node i covers leaves [min(i, j), max(i, j)]
     parent[lc] = (uint32_t)i;
     parent[rc] = (uint32_t)i;
     if (i == 0) parent[0] = NONE;
@@ -225,28 +226,90 @@ __global__ void __launch_bounds__(256) leaf_kernel(const cr_photon_beam* beams, 
 }
 
 // Bottom-up union (PPB_CalculateAABB.comp:100-120, corrected D22): the first
-// lane to arrive at a node stops; the second unions both children.  The
-// counter RMW is acq_rel at agent scope, so the box stores of the first lane
-// (made before its release) are visible to the second lane's loads (made
-// after its acquire), on every XCD.
-__global__ void __launch_bounds__(256) refit_kernel(uint32_t n, const uint32_t* parent, const uint32_t* left,
-                                                    const uint32_t* right, float* lo, float* hi, uint32_t* arrivals)
+// lane to arrive at a node stops; the second unions both children.  Two
+// launches, so no lane ever waits for one on another XCD:
+//  * refit_local_kernel: a workgroup takes REFIT_LEAVES consecutive leaves.
+//    Internal node i covers leaves [min(i, j), max(i, j)] (j = ends[i]); a
+//    node whose leaves all belong to the workgroup has both subtrees inside
+//    it, so its arrival counter and box live in LDS (workgroup-scope RMW).
+//    A lane that completes a child of a node spanning several workgroups
+//    appends that node to `exits` and stops.
+//  * refit_top_kernel: one workgroup climbs from every exit through the
+//    spanning nodes (the ancestors of the workgroup boundaries, a few per
+//    workgroup of the first launch), with workgroup-scope RMWs on global
+//    counters; the boxes of their children are the first launch's output.
+constexpr uint32_t REFIT_LEAVES = 2048;
+constexpr int REFIT_THREADS = 256;
+
+__global__ void __launch_bounds__(REFIT_THREADS) refit_local_kernel(uint32_t n, const uint32_t* parent, const uint32_t* left,
+                                                                    const uint32_t* right, const uint32_t* ends, float* lo,
+                                                                    float* hi, uint32_t* exits, uint32_t* exitCount)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    uint32_t cur = parent[(n - 1) + i];
-    // A Karras tree over 30-bit codes with index tie-breaks is at most 64
-    // levels deep; the bound only protects against a malformed tree.
-    for (int level = 0; cur != NONE && cur < n - 1 && level < 128; level++) {
-        const uint32_t old = __hip_atomic_fetch_add(arrivals + cur, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;
-        const uint32_t a = left[cur], b = right[cur];
+    __shared__ uint32_t arrived[REFIT_LEAVES];
+    __shared__ float blo[REFIT_LEAVES][3], bhi[REFIT_LEAVES][3];
+    const uint32_t first = blockIdx.x * REFIT_LEAVES;
+    for (uint32_t t = threadIdx.x; t < REFIT_LEAVES; t += REFIT_THREADS) arrived[t] = 0u;
+    __syncthreads();
+    const uint32_t inner = n - 1;
+    for (uint32_t leaf = first + threadIdx.x; leaf < first + REFIT_LEAVES && leaf < n; leaf += REFIT_THREADS) {
+        uint32_t cur = parent[inner + leaf];
+        // A Karras tree over 30-bit codes with index tie-breaks is at most 64
+        // levels deep; the bound only protects against a malformed tree.
+        for (int level = 0; cur != NONE && cur < inner && level < 128; level++) {
+            const uint32_t e = ends[cur];
+            if (!(min(cur, e) >= first && max(cur, e) < first + REFIT_LEAVES)) {
+                exits[atomicAdd(exitCount, 1u)] = cur;
+                break;
+            }
+            const uint32_t old =
+                __hip_atomic_fetch_add(&arrived[cur - first], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;
+            const uint32_t a = left[cur], b = right[cur];
+            // a child of a workgroup-local node is a leaf (box from the leaf
+            // kernel) or itself local (box in LDS); the LDS index is formed only
+            // for such a child, so it always lies in [0, REFIT_LEAVES)
+            const bool aL = a < inner, bL = b < inner;
+            const uint32_t ia = aL ? a - first : 0u, ib = bL ? b - first : 0u;
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            lo[3 * (size_t)cur + k] = fminf(lo[3 * (size_t)a + k], lo[3 * (size_t)b + k]);
-            hi[3 * (size_t)cur + k] = fmaxf(hi[3 * (size_t)a + k], hi[3 * (size_t)b + k]);
+            for (int k = 0; k < 3; k++) {
+                const float la = aL ? blo[ia][k] : lo[3 * (size_t)a + k];
+                const float ha = aL ? bhi[ia][k] : hi[3 * (size_t)a + k];
+                const float lb = bL ? blo[ib][k] : lo[3 * (size_t)b + k];
+                const float hb = bL ? bhi[ib][k] : hi[3 * (size_t)b + k];
+                const float l = fminf(la, lb), h = fmaxf(ha, hb);
+                lo[3 * (size_t)cur + k] = l;
+                hi[3 * (size_t)cur + k] = h;
+                blo[cur - first][k] = l;
+                bhi[cur - first][k] = h;
+            }
+            cur = parent[cur];
         }
-        cur = parent[cur];
+    }
+}
+
+constexpr int REFIT_TOP_THREADS = 1024;
+
+__global__ void __launch_bounds__(REFIT_TOP_THREADS) refit_top_kernel(uint32_t n, const uint32_t* parent,
+                                                                      const uint32_t* left, const uint32_t* right,
+                                                                      float* lo, float* hi, const uint32_t* exits,
+                                                                      const uint32_t* exitCount, uint32_t* arrivals)
+{
+    const uint32_t inner = n - 1;
+    const uint32_t count = *exitCount;
+    for (uint32_t x = threadIdx.x; x < count; x += REFIT_TOP_THREADS) {
+        uint32_t cur = exits[x];
+        for (int level = 0; cur != NONE && cur < inner && level < 128; level++) {
+            const uint32_t old =
+                __hip_atomic_fetch_add(arrivals + cur, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0u) break;
+            const uint32_t a = left[cur], b = right[cur];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                lo[3 * (size_t)cur + k] = fminf(lo[3 * (size_t)a + k], lo[3 * (size_t)b + k]);
+                hi[3 * (size_t)cur + k] = fmaxf(hi[3 * (size_t)a + k], hi[3 * (size_t)b + k]);
+            }
+            cur = parent[cur];
+        }
     }
 }
 
@@ -340,10 +403,11 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
 }
 
 hipError_t launch_hierarchy(const uint32_t* keys, uint32_t n, uint32_t* parent, uint32_t* left, uint32_t* right,
-                            hipStream_t s)
+                            uint32_t* ends, hipStream_t s)
 {
     if (n < 2) return hipSuccess;
-    hipLaunchKernelGGL(hierarchy_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, keys, (int32_t)n, parent, left, right);
+    hipLaunchKernelGGL(hierarchy_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s, keys, (int32_t)n, parent, left, right,
+                       ends);
     return hipGetLastError();
 }
 
@@ -355,11 +419,14 @@ hipError_t launch_leaves(const cr_photon_beam* beams, const uint32_t* perm, cons
     return hipGetLastError();
 }
 
-hipError_t launch_refit(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, float* lo, float* hi,
-                        uint32_t* arrivals, hipStream_t s)
+hipError_t launch_refit(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, const uint32_t* ends,
+                        float* lo, float* hi, uint32_t* arrivals, uint32_t* exits, uint32_t* exitCount, hipStream_t s)
 {
     if (n < 2) return hipSuccess;
-    hipLaunchKernelGGL(refit_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, parent, left, right, lo, hi, arrivals);
+    hipLaunchKernelGGL(refit_local_kernel, dim3(blocks_for(n, REFIT_LEAVES)), dim3(REFIT_THREADS), 0, s, n, parent, left,
+                       right, ends, lo, hi, exits, exitCount);
+    hipLaunchKernelGGL(refit_top_kernel, dim3(1), dim3(REFIT_TOP_THREADS), 0, s, n, parent, left, right, lo, hi, exits,
+                       exitCount, arrivals);
     return hipGetLastError();
 }
 
