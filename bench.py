@@ -332,11 +332,11 @@ def bench_c3(args, pkg, cfg):
     # gather: 1080p x --c3-gather-spp frames in one call
     frames = cfg.frames(max(1, args.c3_gather_spp))
     ext = S.GatherExt(0, 0, 0.9, 0)
-    ctx.clear_image()
-    ctx.gather(frames[:1], ext)      # warm-up frame
     gather = None
-    ctx.synchronize()
     if args.c3_gather_spp > 0:
+        ctx.clear_image()
+        ctx.gather(frames[:1], ext)      # warm-up frame
+        ctx.synchronize()
         ctx.clear_image()
         g0 = ctx.gather_stats()
         ctx.synchronize()
