@@ -29,6 +29,8 @@ static_assert(sizeof(TravNode) == 64, "TravNode is one 64-B line");
 hipError_t launch_morton(const cr_photon_beam* beams, uint32_t n, const BoxArg& box, uint32_t* keys, uint32_t* vals,
                          hipStream_t s);
 uint32_t sort_tiles(uint32_t n);
+// words of the counts buffer launch_radix_sort needs for n keys (histograms and digit totals)
+uint32_t sort_counts_words(uint32_t n);
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* counts,
                              uint32_t n, hipStream_t s);
 // ends[i]: the other end j of internal node i's leaf range [min(i, j), max(i, j)]

@@ -210,7 +210,7 @@ cr_status cr_lbvh_build(cr_context* ctx)
     CR_TRY(ensure_buffer(ctx, (void**)&p->vals, &p->valsCap, (size_t)n * 4, "vals"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->keysTmp, &p->keysTmpCap, (size_t)n * 4, "keys tmp"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->valsTmp, &p->valsTmpCap, (size_t)n * 4, "vals tmp"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->counts, &p->countsCap, (size_t)sort_tiles(n) * 256 * 4, "histograms"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->counts, &p->countsCap, (size_t)sort_counts_words(n) * 4, "histograms"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->parent, &p->parentCap, nodes * 4, "parent"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->left, &p->leftCap, (size_t)n * 4, "left"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->right, &p->rightCap, (size_t)n * 4, "right"));

@@ -63,9 +63,9 @@ __global__ void __launch_bounds__(256) morton_kernel(const cr_photon_beam* beams
     vals[i] = i;
 }
 
-// (1) per-tile digit histograms, digit-major.
+// (1) per-tile digit histograms, digit-major, and each digit's total.
 __global__ void __launch_bounds__(256) hist_kernel(const uint32_t* keys, uint32_t n, uint32_t shift, uint32_t tiles,
-                                                   uint32_t* counts)
+                                                   uint32_t* counts, uint32_t* totals)
 {
     __shared__ uint32_t h[4][256];
     const uint32_t w = threadIdx.x >> 6;
@@ -79,31 +79,50 @@ __global__ void __launch_bounds__(256) hist_kernel(const uint32_t* keys, uint32_
     }
     __syncthreads();
     const uint32_t d = threadIdx.x;
-    counts[(size_t)d * tiles + blockIdx.x] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+    const uint32_t c = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+    counts[(size_t)d * tiles + blockIdx.x] = c;
+    if (c) atomicAdd(totals + d, c);
 }
 
-// (2) exclusive scan of m counts in place, one block of 1024 threads.
-__global__ void __launch_bounds__(1024) scan_kernel(uint32_t* c, uint32_t m)
+__global__ void __launch_bounds__(256) zero_totals_kernel(uint32_t* totals) { totals[blockIdx.x * 256 + threadIdx.x] = 0u; }
+
+// Inclusive prefix sum over a 256-thread block (4 waves).
+__device__ __forceinline__ uint32_t block_scan256(uint32_t v, uint32_t* wsum)
 {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (m + 1023) / 1024;
-    const uint32_t lo = t * per, hi = min(lo + per, m);
-    uint32_t s = 0;
-    for (uint32_t i = lo; i < hi; i++) s += c[i];
-    part[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        uint32_t v = (t >= off) ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)v, off, 64);
+        if (lane >= (uint32_t)off) v += o;
     }
-    uint32_t run = part[t] - s;  // exclusive prefix of this thread's chunk
-    for (uint32_t i = lo; i < hi; i++) {
-        uint32_t v = c[i];
-        c[i] = run;
-        run += v;
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    for (uint32_t q = 0; q < w; q++) v += wsum[q];
+    __syncthreads();
+    return v;
+}
+
+// (2) exclusive scan of the digit-major counts in place: block d scans the
+// tiles' counts of digit d, offset by the totals of the digits below d.
+__global__ void __launch_bounds__(256) digit_scan_kernel(uint32_t* counts, uint32_t tiles, const uint32_t* totals)
+{
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t carry;
+    const uint32_t d = blockIdx.x, t = threadIdx.x;
+    const uint32_t below = block_scan256(t < d ? totals[t] : 0u, wsum);
+    if (t == 255) carry = below;  // sum of totals[0 .. d-1]
+    __syncthreads();
+    uint32_t run = carry;
+    uint32_t* row = counts + (size_t)d * tiles;
+    for (uint32_t c0 = 0; c0 < tiles; c0 += 256) {
+        const uint32_t i = c0 + t;
+        const uint32_t x = i < tiles ? row[i] : 0u;
+        const uint32_t incl = block_scan256(x, wsum);
+        if (i < tiles) row[i] = run + incl - x;
+        if (t == 255) carry = run + incl;
+        __syncthreads();
+        run = carry;
+        __syncthreads();
     }
 }
 
@@ -231,22 +250,22 @@ __global__ void __launch_bounds__(256) leaf_kernel(const cr_photon_beam* beams, 
 //  * refit_local_kernel: a workgroup takes REFIT_LEAVES consecutive leaves.
 //    Internal node i covers leaves [min(i, j), max(i, j)] (j = ends[i]); a
 //    node whose leaves all belong to the workgroup has both subtrees inside
-//    it, so its arrival counter and box live in LDS (workgroup-scope RMW).
-//    A lane that completes a child of a node spanning several workgroups
-//    appends that node to `exits` and stops.
+//    it, so its arrival counter lives in LDS and a workgroup-scope acq_rel
+//    RMW orders the box stores and loads of the workgroup's lanes.  A lane
+//    that completes a child of a node spanning several workgroups appends
+//    that node to `exits` and stops.
 //  * refit_top_kernel: one workgroup climbs from every exit through the
 //    spanning nodes (the ancestors of the workgroup boundaries, a few per
 //    workgroup of the first launch), with workgroup-scope RMWs on global
 //    counters; the boxes of their children are the first launch's output.
-constexpr uint32_t REFIT_LEAVES = 2048;
-constexpr int REFIT_THREADS = 256;
+constexpr uint32_t REFIT_LEAVES = 16384;  // 64 KiB of LDS counters
+constexpr int REFIT_THREADS = 1024;
 
 __global__ void __launch_bounds__(REFIT_THREADS) refit_local_kernel(uint32_t n, const uint32_t* parent, const uint32_t* left,
                                                                     const uint32_t* right, const uint32_t* ends, float* lo,
                                                                     float* hi, uint32_t* exits, uint32_t* exitCount)
 {
     __shared__ uint32_t arrived[REFIT_LEAVES];
-    __shared__ float blo[REFIT_LEAVES][3], bhi[REFIT_LEAVES][3];
     const uint32_t first = blockIdx.x * REFIT_LEAVES;
     for (uint32_t t = threadIdx.x; t < REFIT_LEAVES; t += REFIT_THREADS) arrived[t] = 0u;
     __syncthreads();
@@ -265,22 +284,10 @@ __global__ void __launch_bounds__(REFIT_THREADS) refit_local_kernel(uint32_t n, 
                 __hip_atomic_fetch_add(&arrived[cur - first], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0u) break;
             const uint32_t a = left[cur], b = right[cur];
-            // a child of a workgroup-local node is a leaf (box from the leaf
-            // kernel) or itself local (box in LDS); the LDS index is formed only
-            // for such a child, so it always lies in [0, REFIT_LEAVES)
-            const bool aL = a < inner, bL = b < inner;
-            const uint32_t ia = aL ? a - first : 0u, ib = bL ? b - first : 0u;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const float la = aL ? blo[ia][k] : lo[3 * (size_t)a + k];
-                const float ha = aL ? bhi[ia][k] : hi[3 * (size_t)a + k];
-                const float lb = bL ? blo[ib][k] : lo[3 * (size_t)b + k];
-                const float hb = bL ? bhi[ib][k] : hi[3 * (size_t)b + k];
-                const float l = fminf(la, lb), h = fmaxf(ha, hb);
-                lo[3 * (size_t)cur + k] = l;
-                hi[3 * (size_t)cur + k] = h;
-                blo[cur - first][k] = l;
-                bhi[cur - first][k] = h;
+                lo[3 * (size_t)cur + k] = fminf(lo[3 * (size_t)a + k], lo[3 * (size_t)b + k]);
+                hi[3 * (size_t)cur + k] = fmaxf(hi[3 * (size_t)a + k], hi[3 * (size_t)b + k]);
             }
             cur = parent[cur];
         }
@@ -380,6 +387,7 @@ hipError_t launch_morton(const cr_photon_beam* beams, uint32_t n, const BoxArg& 
 }
 
 uint32_t sort_tiles(uint32_t n) { return (n + TILE - 1) / TILE; }
+uint32_t sort_counts_words(uint32_t n) { return sort_tiles(n) * 256u + 4u * 256u; }
 
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* counts,
                              uint32_t n, hipStream_t s)
@@ -390,10 +398,14 @@ hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
     uint32_t* vi = vals;
     uint32_t* ko = keys_tmp;
     uint32_t* vo = vals_tmp;
+    // each pass's digit totals follow the histograms
+    uint32_t* totals = counts + (size_t)tiles * 256u;
+    hipLaunchKernelGGL(zero_totals_kernel, dim3(4), dim3(256), 0, s, totals);
     for (uint32_t pass = 0; pass < 4; pass++) {
         const uint32_t shift = 8u * pass;
-        hipLaunchKernelGGL(hist_kernel, dim3(tiles), dim3(256), 0, s, ki, n, shift, tiles, counts);
-        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, counts, tiles * 256u);
+        uint32_t* tot = totals + 256u * pass;
+        hipLaunchKernelGGL(hist_kernel, dim3(tiles), dim3(256), 0, s, ki, n, shift, tiles, counts, tot);
+        hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(256), 0, s, counts, tiles, tot);
         hipLaunchKernelGGL(scatter_kernel, dim3(tiles), dim3(256), 0, s, ki, vi, n, shift, tiles, counts, ko, vo);
         uint32_t* t = ki; ki = ko; ko = t;
         t = vi; vi = vo; vo = t;

@@ -17,7 +17,7 @@ __global__ void __launch_bounds__(256) order_keys_kernel(const uint32_t* cost, u
 
 namespace cr {
 
-uint32_t pt_order_counts(uint32_t n) { return ((n + 255u) / 256u + 1u) * 256u * 4u; }
+uint32_t pt_order_counts(uint32_t n) { return sort_counts_words(n) * 4u; }
 
 hipError_t launch_tile_order(const uint32_t* cost, uint32_t n, uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp,
                              uint32_t* vals_tmp, uint32_t* counts, hipStream_t s)
