@@ -201,6 +201,22 @@ __device__ __forceinline__ void beam_radiance(const cr::GatherArgs& a, uint32_t 
 {
     const cr_photon_beam B = a.beams[bi];
     const f3 st = ld3(B.startPos), en = ld3(B.endPos);
+    {
+        // Skip a beam whose line passes the camera ray's line farther than its
+        // radius plus a margin: the test below measures the distance between a
+        // point of the camera ray and a point of the beam's line, never less
+        // than the distance between the lines, so it would reject the beam too.
+        // The margin (1e-4 of the coordinates' magnitude, plus 1e-3 of the
+        // radius) exceeds the rounding of both computations many times over;
+        // a NaN anywhere keeps the beam for the full test.
+        const f3 u = sub(en, st), w = sub(st, ro);
+        const f3 nrm = cross(rd, u);
+        const float n2 = dot(nrm, nrm), wn = dot(w, nrm);
+        const float mag = fmaxf(fmaxf(fmaxf(fabsf(ro.x), fabsf(ro.y)), fmaxf(fabsf(ro.z), fabsf(st.x))),
+                                fmaxf(fmaxf(fabsf(st.y), fabsf(st.z)), fmaxf(fmaxf(fabsf(en.x), fabsf(en.y)), fabsf(en.z))));
+        const float r = B.radius * 1.001f + 1e-4f * (mag + 1.0f);
+        if (wn * wn > (r * r) * n2) return;
+    }
     const f3 bd = normalize(sub(en, st));
     if (fabsf(dot(rd, bd)) > 0.9f) return;
     const f3 tangent = cross(rd, bd);
