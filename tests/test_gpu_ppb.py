@@ -160,3 +160,23 @@ def test_gather_bvh_equals_brute_force(ctx, cfg, S, O):
     for (x, y) in [(24, 16), (10, 20), (30, 8), (20, 25)]:
         brute = O.gather_pixel(b, x, y, fr[0].seed, brute=True)
         np.testing.assert_allclose(gpu[y, x], brute, rtol=2e-4, atol=1e-6)
+
+
+def test_lbvh_full_size_bitexact(ctx, cfg, S, O):
+    """The photon-beam workload at full size: 2^22 beams traced on the c1
+    volume (seed 38, radius 0.002 x box side); sorted keys, permutation and all
+    2^23 - 1 nodes equal the oracle's build of the same beams."""
+    sc = cfg.c1(width=64, height=64)
+    cloud, _ = sc.setup(ctx, build_sv=False)
+    side = max(cloud.bounds[1][k] - cloud.bounds[0][k] for k in range(3))
+    p = cfg.beam_params(1 << 19, seed=38, radius=0.002 * side, albedo=0.9, light_dir=sc.light_dir, max_interactions=64)
+    nb, nd = ctx.trace_beams(p, 1 << 22)
+    assert nb == 1 << 22
+    ctx.lbvh_build()
+    keys, perm, nodes = ctx.lbvh_download()
+    beams, _ = ctx.download_beams()
+    rk, rp, rn = O.lbvh_build(beams, cloud.bounds[0], cloud.bounds[1])
+    np.testing.assert_array_equal(keys, rk, err_msg="sorted keys")
+    np.testing.assert_array_equal(perm, rp, err_msg="permutation")
+    assert_same_tree(nodes, rn, nb)
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
