@@ -6,8 +6,10 @@
 //
 // Radix sort: 4 passes of 8-bit digits.  Each pass is
 //   (1) tile histograms: 4096 keys per 256-thread block, per-wave LDS
-//       histograms, written digit-major (digit * tiles + tile);
-//   (2) one exclusive scan over the 256 x tiles counts (single block);
+//       histograms, written digit-major (digit * tiles + tile), and each
+//       digit's total (global atomics);
+//   (2) exclusive scan of the counts: one workgroup per digit scans that
+//       digit's tiles, offset by the totals of the smaller digits;
 //   (3) stable scatter: the tile is consumed in rounds of 256 keys (one key
 //       per lane, coalesced); within a wave, lanes with equal digits are found
 //       with 8 ballots (a wave-level multisplit), their rank is the popcount
@@ -15,9 +17,10 @@
 //       previous rounds of the tile, ranks are offset by LDS counters.  The
 //       result is the unique stable order "by key, ties by input index".
 // Hierarchy: one lane per internal node, Karras 2012 Fig. 4 with the
-// index tie-break for equal codes.  Refit: one lane per leaf walks up; the
-// second lane to reach a node (agent-scope acq_rel counter) unions the two
-// child boxes.
+// index tie-break for equal codes; it also records each node's leaf range.
+// Refit: one lane per leaf walks up; the second lane to reach a node unions
+// the two child boxes — first inside workgroups of consecutive leaves (LDS
+// counters), then, in a second launch, through the nodes spanning them.
 #include "cr_device.h"
 #include "cr_internal_ppb.h"
 
