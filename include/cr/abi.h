@@ -134,7 +134,8 @@ CR_API cr_status cr_grid_majorants(cr_context* ctx, uint32_t dims_out[3], float*
 /* The empty-space skip grid the brick walk reads (extension): for a non-empty
  * brick its extinction majorant (majorant * densityScaling) / baseScaling, for
  * an empty brick -d, d being the max-norm distance in bricks to the nearest
- * non-empty brick (d <= 16; 16 = at least 16). */
+ * non-empty brick (d <= 16; 16 = at least 16).  The walk itself uses only the
+ * sign (an empty brick is crossed without a lookup). */
 CR_API cr_status cr_grid_skip(cr_context* ctx, float* out);
 
 /* Seeded value-noise fBm cloud generated on the device (extension; the
