@@ -25,9 +25,22 @@ struct MlpArgs {
     uint16_t* acts;          // [4][B][128] bf16 hidden activations (training)
     size_t actStride;        // elements between layers in acts
     const float* dY;         // [B][4] fp32
-    float* grad;             // flat fp32 gradients (same layout as the parameters)
+    float* grad;             // flat fp32 gradients (same layout as the parameters), written by the Adam kernel
+    float* partial;          // [nWaves][total] per-wave gradient sums of the backward pass
+    uint32_t nWaves;         // backward waves: wave w takes the 32-row chunks w, w + nWaves, ...
+    uint32_t total;          // flat parameter count
     uint32_t wOff[MLP_LAYERS], bOff[MLP_LAYERS];
 };
+
+// Backward waves for a batch of B rows: one per 32-row chunk, at most MLP_MAX_WAVES.
+constexpr uint32_t MLP_MAX_WAVES = 1024;
+inline uint32_t mlp_waves(uint64_t B)
+{
+    const uint64_t c = (B + 31) / 32;
+    return (uint32_t)(c < MLP_MAX_WAVES ? c : MLP_MAX_WAVES);
+}
+// Loss kernel blocks (one partial loss each) for a batch of B rows.
+inline unsigned mlp_loss_blocks(int64_t B) { return (unsigned)((B + 255) / 256); }
 
 struct AdamArgs {
     float* params;
@@ -39,6 +52,7 @@ struct AdamArgs {
 
 hipError_t launch_mlp_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);
 hipError_t launch_mlp_forward(const MlpArgs& a, bool train, hipStream_t s);
+// loss: mlp_loss_blocks(B) partial sums, to be added in order
 hipError_t launch_mlp_loss(const float* Y, const float* T, int64_t B, float* dY, float* loss, hipStream_t s);
 hipError_t launch_mlp_backward(const MlpArgs& a, hipStream_t s);
 hipError_t launch_mlp_adam(const MlpArgs& a, const AdamArgs& o, hipStream_t s);

@@ -33,8 +33,11 @@ struct Mlp {
     size_t dYCap = 0;
     uint16_t* acts = nullptr;
     size_t actsCap = 0;
-    float* loss = nullptr;
+    float* loss = nullptr;     // the last training step's per-block loss sums
     size_t lossCap = 0;
+    unsigned lossBlocks = 0;
+    float* partial = nullptr;  // per-wave gradient sums of the backward pass
+    size_t partialCap = 0;
 };
 
 namespace {
@@ -119,7 +122,10 @@ cr_status ensure_batch(cr_context* ctx, Mlp* m, uint64_t B, bool train)
 {
     cr_status st = ensure_buffer(ctx, (void**)&m->X, &m->XCap, B * 224 * 2, "mlp X");
     if (st == CR_OK) st = ensure_buffer(ctx, (void**)&m->Y, &m->YCap, B * 16, "mlp Y");
-    if (st == CR_OK) st = ensure_buffer(ctx, (void**)&m->loss, &m->lossCap, 64, "mlp loss");
+    if (st == CR_OK && train)
+        st = ensure_buffer(ctx, (void**)&m->loss, &m->lossCap, (size_t)mlp_loss_blocks((int64_t)B) * 4, "mlp loss");
+    if (st == CR_OK && train)
+        st = ensure_buffer(ctx, (void**)&m->partial, &m->partialCap, (size_t)mlp_waves(B) * m->total * 4, "mlp partial gradients");
     if (st == CR_OK && train) st = ensure_buffer(ctx, (void**)&m->T, &m->TCap, B * 12, "mlp targets");
     if (st == CR_OK && train) st = ensure_buffer(ctx, (void**)&m->dY, &m->dYCap, B * 16, "mlp dY");
     if (st == CR_OK && train) st = ensure_buffer(ctx, (void**)&m->acts, &m->actsCap, B * 128 * 2 * 4, "mlp acts");
@@ -144,6 +150,9 @@ MlpArgs args_of(Mlp* m, const uint16_t* X, uint64_t B)
     a.actStride = (size_t)B * 128;
     a.dY = m->dY;
     a.grad = m->grad;
+    a.partial = m->partial;
+    a.nWaves = mlp_waves(B);
+    a.total = m->total;
     return a;
 }
 
@@ -154,7 +163,7 @@ void mlp_free(cr_context* ctx)
 {
     Mlp* m = ctx->mlp;
     if (!m) return;
-    void* bufs[] = {m->params, m->grad, m->m, m->v, m->X, m->T, m->Y, m->dY, m->acts, m->loss};
+    void* bufs[] = {m->params, m->grad, m->m, m->v, m->X, m->T, m->Y, m->dY, m->acts, m->loss, m->partial};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (int l = 0; l < MLP_LAYERS; l++) {
@@ -295,9 +304,7 @@ cr_status cr_mlp_train_step(cr_context* ctx, const uint16_t* X, const float* tar
     CR_HIP(ctx, hipMemcpyAsync(m->X, X, (size_t)B * 224 * 2, hipMemcpyHostToDevice, s), "copy X");
     CR_HIP(ctx, hipMemcpyAsync(m->T, targets, (size_t)B * 12, hipMemcpyHostToDevice, s), "copy targets");
     CR_TRY(mlp_train_device(ctx, m->X, m->T, B, lr));
-    if (loss) {
-        CR_HIP(ctx, hipMemcpyAsync(loss, m->loss, 4, hipMemcpyDeviceToHost, s), "copy loss");
-    }
+    if (loss) CR_TRY(mlp_last_loss(ctx, loss));
     if (grads) {
         CR_HIP(ctx, hipMemcpyAsync(grads, m->grad, (size_t)m->total * 4, hipMemcpyDeviceToHost, s), "copy grads");
     }
@@ -328,13 +335,19 @@ cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t* B, fl
 
 namespace cr {
 
-// Loss of the last training step (synchronises the stream).
+// Loss of the last training step: its blocks' sums added in block order
+// (synchronises the stream).
 cr_status mlp_last_loss(cr_context* ctx, float* loss)
 {
     Mlp* m;
     CR_TRY(mlp_get(ctx, &m));
-    CR_HIP(ctx, hipMemcpyAsync(loss, m->loss, 4, hipMemcpyDeviceToHost, ctx->stream), "copy loss");
+    std::vector<float> part(m->lossBlocks);
+    if (m->lossBlocks)
+        CR_HIP(ctx, hipMemcpyAsync(part.data(), m->loss, part.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "copy loss");
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    float sum = 0.0f;
+    for (float v : part) sum += v;
+    *loss = sum;
     return CR_OK;
 }
 
@@ -346,10 +359,9 @@ cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, u
     CR_TRY(ensure_batch(ctx, m, B, true));
     hipStream_t s = ctx->stream;
     MlpArgs a = args_of(m, X, B);
-    CR_HIP(ctx, zero_async(m->loss, 4, s), "zero loss");
-    CR_HIP(ctx, zero_async(m->grad, (size_t)m->total * 4, s), "zero grad");
     CR_HIP(ctx, launch_mlp_forward(a, true, s), "mlp forward");
     CR_HIP(ctx, launch_mlp_loss(m->Y, T, (int64_t)B, m->dY, m->loss, s), "mlp loss");
+    m->lossBlocks = mlp_loss_blocks((int64_t)B);
     CR_HIP(ctx, launch_mlp_backward(a, s), "mlp backward");
     m->step++;
     AdamArgs o;

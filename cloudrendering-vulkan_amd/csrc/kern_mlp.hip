@@ -17,8 +17,10 @@
 // Backward: per wave and layer, dW += A_in^T dZ (A^T fragments from LDS, the
 // dZ accumulator tiles are already in B-operand layout), db += column sums of
 // dZ, dA = dZ W^T (dZ staged through LDS into A layout, W read k-major), and
-// dZ_prev = dA * (A_in > 0).  Parameter gradients are accumulated with fp32
-// atomics into one flat buffer.
+// dZ_prev = dA * (A_in > 0).  Every wave sums its parameter gradients into a
+// slot of its own (the flat parameter layout; no atomics), looping over
+// 32-row chunks w, w + nWaves, ...; the Adam kernel adds the slots in wave
+// order, so a training step is bit-reproducible.
 #include "cr_device.h"
 #include "cr_internal_mlp.h"
 
@@ -141,9 +143,11 @@ __global__ void __launch_bounds__(256) mlp_fwd_kernel(cr::MlpArgs a)
     }
 }
 
-// dY = 2 / (B * 3) (Y - log(1 + max(t, 0))), loss = mean squared difference.
+// dY = 2 / (B * 3) (Y - log(1 + max(t, 0))); loss[block] = this block's share
+// of the mean squared difference (the host adds the blocks in order).
 __global__ void __launch_bounds__(256) mlp_loss_kernel(const float* Y, const float* T, int64_t B, float* dY, float* loss)
 {
+    __shared__ float waveSum[4];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     float s = 0.0f;
     if (i < B) {
@@ -157,15 +161,19 @@ __global__ void __launch_bounds__(256) mlp_loss_kernel(const float* Y, const flo
         dY[i * 4 + 3] = 0.0f;
     }
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, s / ((float)B * 3.0f));
+    if ((threadIdx.x & 63) == 0) waveSum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        loss[blockIdx.x] = (((waveSum[0] + waveSum[1]) + waveSum[2]) + waveSum[3]) / ((float)B * 3.0f);
 }
 
 // Backward through one layer for the wave's 32 rows.  dz holds dZ of this
 // layer's output in accumulator layout; on return (li > 0) it holds dZ of the
-// previous layer's output.
+// previous layer's output.  gW / gb: the wave's gradient slot of this layer;
+// `first` (the wave's first chunk) stores instead of adding.
 template <int K, int N>
 __device__ __forceinline__ void layer_bwd(const cr::MlpArgs& a, int li, const uint16_t* ain, uint16_t* dzl, v4f (&dz)[2][8],
-                                          float* gW, float* gb)
+                                          float* gW, float* gb, bool first)
 {
     const int l = threadIdx.x & 63;
     // (a) dW[k][n] += sum_rows ain[row][k] dz[row][n]
@@ -189,7 +197,8 @@ __device__ __forceinline__ void layer_bwd(const cr::MlpArgs& a, int li, const ui
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int k = mt * 16 + 4 * (l / 16) + j, n = nt * 16 + l % 16;
-                atomicAdd(gW + (size_t)k * N + n, c[j]);
+                float* g = gW + (size_t)k * N + n;
+                *g = first ? c[j] : *g + c[j];
             }
         }
     }
@@ -203,7 +212,10 @@ __device__ __forceinline__ void layer_bwd(const cr::MlpArgs& a, int li, const ui
             for (int j = 0; j < 4; j++) s += dz[rt][nt][j];
         s += __shfl_xor(s, 16, 64);
         s += __shfl_xor(s, 32, 64);
-        if (l < 16) atomicAdd(gb + nt * 16 + l, s);
+        if (l < 16) {
+            float* g = gb + nt * 16 + l;
+            *g = first ? s : *g + s;
+        }
     }
     if (li == 0) return;
     // (c) dA = dZ W^T: stage dZ (bf16) in LDS as [row][n]
@@ -249,49 +261,60 @@ __global__ void __launch_bounds__(256) mlp_bwd_kernel(cr::MlpArgs a)
     __shared__ uint16_t lds_a[4 * 32 * ROW_STRIDE];
     __shared__ uint16_t lds_dz[4 * 32 * DZ_STRIDE];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * 32;
-    if (row0 >= a.B) return;
+    const uint32_t w = blockIdx.x * 4u + (uint32_t)wave;
+    if (w >= a.nWaves) return;
     uint16_t* ain = lds_a + wave * 32 * ROW_STRIDE;
     uint16_t* dzl = lds_dz + wave * 32 * DZ_STRIDE;
-    v4f dz[2][8];
+    float* slot = a.partial + (size_t)w * a.total;
+    for (int64_t chunk = w; chunk * 32 < a.B; chunk += a.nWaves) {
+        const int64_t row0 = chunk * 32;
+        const bool first = chunk == (int64_t)w;
+        v4f dz[2][8];
 #pragma unroll
-    for (int rt = 0; rt < 2; rt++)
+        for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int64_t r = row0 + rt * 16 + 4 * (l / 16) + j;
-            const int n = l % 16;
-            dz[rt][0][j] = (r < a.B && n < 3) ? a.dY[r * 4 + n] : 0.0f;
-        }
-#pragma unroll
-    for (int li = 4; li >= 0; li--) {
-        // stage this layer's input rows (X or the previous layer's output) in LDS
-        const int width = li == 0 ? 224 : 128;
-        const int chunks = width / 8;
-        for (int idx = l; idx < 32 * chunks; idx += 64) {
-            const int r = idx / chunks, c = idx % chunks;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (row0 + r < a.B) {
-                const uint16_t* src = li == 0 ? a.X + (size_t)(row0 + r) * 224
-                                              : a.acts + (size_t)(li - 1) * a.actStride + (size_t)(row0 + r) * 128;
-                v = *reinterpret_cast<const uint4*>(src + c * 8);
+            for (int j = 0; j < 4; j++) {
+                const int64_t r = row0 + rt * 16 + 4 * (l / 16) + j;
+                const int n = l % 16;
+                dz[rt][0][j] = (r < a.B && n < 3) ? a.dY[r * 4 + n] : 0.0f;
             }
-            *reinterpret_cast<uint4*>(ain + r * ROW_STRIDE + c * 8) = v;
+#pragma unroll
+        for (int li = 4; li >= 0; li--) {
+            // stage this layer's input rows (X or the previous layer's output) in LDS
+            const int width = li == 0 ? 224 : 128;
+            const int chunks = width / 8;
+            for (int idx = l; idx < 32 * chunks; idx += 64) {
+                const int r = idx / chunks, c = idx % chunks;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (row0 + r < a.B) {
+                    const uint16_t* src = li == 0 ? a.X + (size_t)(row0 + r) * 224
+                                                  : a.acts + (size_t)(li - 1) * a.actStride + (size_t)(row0 + r) * 128;
+                    v = *reinterpret_cast<const uint4*>(src + c * 8);
+                }
+                *reinterpret_cast<uint4*>(ain + r * ROW_STRIDE + c * 8) = v;
+            }
+            float* gW = slot + a.wOff[li];
+            float* gb = slot + a.bOff[li];
+            if (li == 4) layer_bwd<128, 16>(a, li, ain, dzl, dz, gW, gb, first);
+            else if (li > 0) layer_bwd<128, 128>(a, li, ain, dzl, dz, gW, gb, first);
+            else layer_bwd<224, 128>(a, li, ain, dzl, dz, gW, gb, first);
         }
-        float* gW = a.grad + a.wOff[li];
-        float* gb = a.grad + a.bOff[li];
-        if (li == 4) layer_bwd<128, 16>(a, li, ain, dzl, dz, gW, gb);
-        else if (li > 0) layer_bwd<128, 128>(a, li, ain, dzl, dz, gW, gb);
-        else layer_bwd<224, 128>(a, li, ain, dzl, dz, gW, gb);
     }
 }
 
-// Adam over the flat fp32 parameters, then refresh the bf16 copies
+// Gradient = the waves' slots added in order; Adam over the flat fp32
+// parameters, then refresh the bf16 copies
 // (k-major W for backward, n-major WT for forward).
 __global__ void __launch_bounds__(256) mlp_adam_kernel(cr::MlpArgs a, cr::AdamArgs o)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= o.n) return;
-    const float g = a.grad[i];
+    // the waves' gradient slots, added in wave order
+    float g = 0.0f;
+    const float* part = a.partial + i;
+#pragma unroll 8
+    for (uint32_t w = 0; w < a.nWaves; w++) g += part[(size_t)w * a.total];
+    a.grad[i] = g;
     const float m = o.beta1 * o.m[i] + (1.0f - o.beta1) * g;
     const float v = o.beta2 * o.v[i] + (1.0f - o.beta2) * g * g;
     o.m[i] = m;
@@ -334,14 +357,14 @@ hipError_t launch_mlp_forward(const MlpArgs& a, bool train, hipStream_t s)
 
 hipError_t launch_mlp_loss(const float* Y, const float* T, int64_t B, float* dY, float* loss, hipStream_t s)
 {
-    hipLaunchKernelGGL(mlp_loss_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, Y, T, B, dY, loss);
+    hipLaunchKernelGGL(mlp_loss_kernel, dim3(mlp_loss_blocks(B)), dim3(256), 0, s, Y, T, B, dY, loss);
     return hipGetLastError();
 }
 
 hipError_t launch_mlp_backward(const MlpArgs& a, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)((a.B + 127) / 128);
-    if (blocks == 0) return hipSuccess;
+    const unsigned blocks = (a.nWaves + 3u) / 4u;
+    if (blocks == 0 || a.B <= 0) return hipSuccess;
     hipLaunchKernelGGL(mlp_bwd_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }

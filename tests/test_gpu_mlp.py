@@ -93,12 +93,14 @@ def test_forward_matches_oracle(ctx, pkg, M):
     assert err < 2e-2, err
 
 
-def test_gradients_match_oracle(ctx, pkg, M):
+# 256 rows: 8 backward waves of one 32-row chunk each; 70000 rows: 1024 waves (the most the
+# backward launches), 140 of which take three chunks (the last one ragged) and the others two
+@pytest.mark.parametrize("B", [256, 70000])
+def test_gradients_match_oracle(ctx, pkg, M, B):
     ps = make_params(M, 3)
     flat = pack(pkg, ps)
     lib = pkg.load()
     ctx.check(lib.cr_mlp_set_params(ctx.h, flat.ctypes.data, len(flat)), "set params")
-    B = 256
     rng = np.random.default_rng(4)
     X = rng.random((B, 224)).astype(np.float32)
     T = rng.random((B, 3)).astype(np.float32) * 3.0
@@ -117,6 +119,34 @@ def test_gradients_match_oracle(ctx, pkg, M):
     K, N, wo, bo, total = layout(pkg)
     pad = grads[wo[4]: wo[4] + K[4] * N[4]].reshape(K[4], N[4])[:, 3:]
     assert not pad.any()
+
+
+def test_training_is_bit_reproducible(ctx, pkg, M):
+    """The backward pass sums gradients per wave and the Adam kernel adds the
+    waves in order (no float atomics), so the same steps from the same
+    parameters give bit-identical gradients, losses and parameters."""
+    ps = make_params(M, 9)
+    flat = pack(pkg, ps)
+    lib = pkg.load()
+    B = 5000
+    rng = np.random.default_rng(10)
+    Xb, _ = bf16(rng.random((B, 224)).astype(np.float32))
+    T = (rng.random((B, 3)) * 2.0).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        ctx.check(lib.cr_mlp_set_params(ctx.h, flat.ctypes.data, len(flat)), "set params")
+        losses, grads = [], np.zeros(len(flat), np.float32)
+        loss = C.c_float()
+        for _ in range(5):
+            ctx.check(lib.cr_mlp_train_step(ctx.h, Xb.ctypes.data, T.ctypes.data, B, 1e-3, C.byref(loss),
+                                            grads.ctypes.data), "train step")
+            losses.append(loss.value)
+        after = np.zeros(len(flat), np.float32)
+        ctx.check(lib.cr_mlp_get_params(ctx.h, after.ctypes.data, len(after)), "get params")
+        runs.append((np.array(losses, np.float32), grads.copy(), after))
+    for (a, b, what) in zip(runs[0], runs[1], ("losses", "gradients", "parameters")):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), what
+    assert not np.array_equal(runs[0][2], flat)
 
 
 def test_adam_step_matches_oracle(ctx, pkg, M):
