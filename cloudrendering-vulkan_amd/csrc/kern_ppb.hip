@@ -182,9 +182,21 @@ __global__ void __launch_bounds__(256) trace_kernel(cr::TraceArgs a)
     }
 }
 
+// A ray with a direction component exactly 0 keeps that coordinate for every
+// t, so it misses every box whose slab on that axis does not hold it; the
+// slab formula cannot tell (m = inf gives inf - inf = NaN, which fmaxf / fminf
+// drop), so without this test such a ray culls nothing on that axis (the
+// centre pixel's ray, direction (0, 0, 1), would visit the whole tree).  The
+// margin covers the rounding of the center / half-size form of the box.
+__device__ __forceinline__ bool outside_slab(float ro, float d, float cen, float rad)
+{
+    return d == 0.0f && fabsf(ro - cen) > rad + 1e-6f * (fabsf(ro) + fabsf(cen) + rad);
+}
+
 // intersectBox (PPB_PE.comp:282-300) with rad = half the box size; the box as
-// center and half extent, m = 1 / rd formed once per ray.
-__device__ __forceinline__ bool hit_box(f3 ro, f3 m, const float* cen3, const float* rad3)
+// center and half extent, m = 1 / rd formed once per ray; plus the test of
+// the axes the ray is parallel to.
+__device__ __forceinline__ bool hit_box(f3 ro, f3 rd, f3 m, const float* cen3, const float* rad3)
 {
     const f3 cen = mk(cen3[0], cen3[1], cen3[2]);
     const f3 rad = mk(rad3[0], rad3[1], rad3[2]);
@@ -192,7 +204,9 @@ __device__ __forceinline__ bool hit_box(f3 ro, f3 m, const float* cen3, const fl
     const f3 k = mk(fabsf(m.x) * rad.x, fabsf(m.y) * rad.y, fabsf(m.z) * rad.z);
     const float tN = fmaxf(fmaxf(-n.x - k.x, -n.y - k.y), -n.z - k.z);
     const float tF = fminf(fminf(-n.x + k.x, -n.y + k.y), -n.z + k.z);
-    return !(tN > tF || tF < 0.0f);
+    const bool parallelMiss = outside_slab(ro.x, rd.x, cen.x, rad.x) || outside_slab(ro.y, rd.y, cen.y, rad.y) ||
+                              outside_slab(ro.z, rd.z, cen.z, rad.z);
+    return !(tN > tF || tF < 0.0f || parallelMiss);
 }
 
 // accumulateRadiance — PPB_PE.comp:362-404
@@ -303,7 +317,7 @@ __global__ void __launch_bounds__(GATHER_THREADS) gather_kernel(cr::GatherArgs a
                                       (a.leafLo[2] + a.leafHi[2]) * 0.5f};
                 const float rad[3] = {(a.leafHi[0] - a.leafLo[0]) * 0.5f, (a.leafHi[1] - a.leafLo[1]) * 0.5f,
                                       (a.leafHi[2] - a.leafLo[2]) * 0.5f};
-                if (hit_box(ro, m, cen, rad)) {
+                if (hit_box(ro, rd, m, cen, rad)) {
                     beam_radiance<T, LAYOUT>(a, 0, ro, rd, L);
                     tested++;
                 }
@@ -322,8 +336,8 @@ __global__ void __launch_bounds__(GATHER_THREADS) gather_kernel(cr::GatherArgs a
                     }
                     visited++;
                     const cr::TravNode nd = a.nodes[cur];
-                    const bool hl = hit_box(ro, m, nd.cen[0], nd.rad[0]);
-                    const bool hr = hit_box(ro, m, nd.cen[1], nd.rad[1]);
+                    const bool hl = hit_box(ro, rd, m, nd.cen[0], nd.rad[0]);
+                    const bool hr = hit_box(ro, rd, m, nd.cen[1], nd.rad[1]);
                     const bool ll = (nd.leafMask & 1u) != 0, lr = (nd.leafMask & 2u) != 0;
                     if (hl && ll) { beam_radiance<T, LAYOUT>(a, nd.child[0] - inner, ro, rd, L); tested++; }
                     if (hr && lr) { beam_radiance<T, LAYOUT>(a, nd.child[1] - inner, ro, rd, L); tested++; }

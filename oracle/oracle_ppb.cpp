@@ -223,7 +223,17 @@ PathOut trace_path(const or_scene& s, const cr_beam_trace_params& P, uint32_t id
     return o;
 }
 
-/* intersectBox — PPB_PE.comp:282-300 with rad = half the box size. */
+/* Deviation: a ray with a direction component exactly 0 misses a box whose slab
+   on that axis does not hold it (by more than the rounding of the center /
+   half-size form).  The slab formula yields NaN there (inf - inf), which fmaxf /
+   fminf drop, so that axis would never cull. */
+inline bool outside_slab(float ro, float d, float cen, float rad)
+{
+    return d == 0.0f && fabsf(ro - cen) > rad + 1e-6f * (fabsf(ro) + fabsf(cen) + rad);
+}
+
+/* intersectBox — PPB_PE.comp:282-300 with rad = half the box size, plus the
+   parallel-axis test above. */
 inline bool hit_box(f3 ro, f3 rd, const float* lo, const float* hi)
 {
     f3 cen = mk((lo[0] + hi[0]) / 2.0f, (lo[1] + hi[1]) / 2.0f, (lo[2] + hi[2]) / 2.0f);
@@ -233,7 +243,9 @@ inline bool hit_box(f3 ro, f3 rd, const float* lo, const float* hi)
     f3 k = mk(fabsf(m.x) * rad.x, fabsf(m.y) * rad.y, fabsf(m.z) * rad.z);
     float tN = fmaxf(fmaxf(-n.x - k.x, -n.y - k.y), -n.z - k.z);
     float tF = fminf(fminf(-n.x + k.x, -n.y + k.y), -n.z + k.z);
-    return !(tN > tF || tF < 0.0f);
+    bool parallelMiss = outside_slab(ro.x, rd.x, cen.x, rad.x) || outside_slab(ro.y, rd.y, cen.y, rad.y) ||
+                        outside_slab(ro.z, rd.z, cen.z, rad.z);
+    return !(tN > tF || tF < 0.0f || parallelMiss);
 }
 
 /* accumulateRadiance — PPB_PE.comp:362-404 */

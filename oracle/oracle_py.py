@@ -264,14 +264,16 @@ class Oracle:
         b.albedo = albedo
         return b
 
-    def gather_pixel(self, b, x, y, seed, brute=False):
+    def gather_pixel(self, b, x, y, seed, brute=False, stats=False):
+        """Radiance of one pixel's ray; with stats, also (nodes visited, beams tested)."""
         out = (C.c_float * 4)()
+        nv, nt = C.c_uint64(), C.c_uint64()
         if brute:
             self.lib.or_gather_pixel_brute(C.byref(b), x, y, seed, out)
         else:
-            nv, nt = C.c_uint64(), C.c_uint64()
             self.lib.or_gather_pixel(C.byref(b), x, y, seed, out, C.byref(nv), C.byref(nt))
-        return np.array(out, np.float32)
+        r = np.array(out, np.float32)
+        return (r, nv.value, nt.value) if stats else r
 
     def gather_render(self, b, frames, accum=None, threads=0):
         H, W = 2 * b.base.cam.halfHeight, 2 * b.base.cam.halfWidth

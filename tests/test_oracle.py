@@ -465,6 +465,30 @@ def test_gather_bvh_equals_brute_force_oracle(O, S, pkg, cfg):
     assert nonzero > 10
 
 
+def test_gather_axis_parallel_rays_cull(O, S, pkg, cfg):
+    """Rays with a direction component exactly 0 (the 12 x 10 camera's centre
+    column, centre row and centre pixel, whose ray is (0, 0, 1)) still cull
+    the boxes they lie outside of: intersectBox's slab formula gives NaN on
+    such an axis, which would let the ray visit every node.  The culled
+    traversal still equals the loop over every beam."""
+    rng = np.random.default_rng(4)
+    g = rng.random((10, 10, 10), dtype=np.float32)
+    c = unit_cloud(S, pkg, g, 8.0)
+    cam = cfg.camera(12, 10, position=(0.0, 0.0, -0.3))
+    svp = cfg.shadow_volume(c, tuple(cam.forward), 8)
+    s = O.scene(g, c, cam, cfg.parameters(0.7), svp, cfg.render_ext(), sv=np.full((8, 8, 8), 0.8, np.float32))
+    p = cfg.beam_params(400, seed=5, radius=0.01, albedo=0.9)
+    beams, data, nb, nd = O.trace_beams(s, p, 1 << 16)
+    assert nb == 1 << 16
+    keys, perm, nodes = O.lbvh_build(beams, c.bounds[0], c.bounds[1])
+    sb = (S.PhotonBeam * nb)(*[beams[int(i)] for i in perm])
+    bs = O.beam_scene(s, sb, data, nodes, nb, 0, 0, 0.9)
+    for (x, y) in [(6, 5), (6, 1), (6, 8), (2, 5), (10, 5)]:
+        r, visited, tested = O.gather_pixel(bs, x, y, 38, stats=True)
+        assert visited < (nb - 1) // 100, ((x, y), visited)
+        np.testing.assert_allclose(r, O.gather_pixel(bs, x, y, 38, brute=True), rtol=1e-4, atol=1e-7)
+
+
 def test_ppb_update_radius_pinned(O):
     """UpdateRadius (RenderTechniquePPB.cpp:530-546) against what its formula
     fixes: restart at frames 0 and 1, a sum of exactly B ones for alpha = 1,
