@@ -50,6 +50,8 @@ def parse():
                     help="dda: brick-majorant tracking + ratio-tracked shadow rays (the production tracker); "
                          "global: the reference's global majorant + shadow-volume NEE (c1's settings)")
     ap.add_argument("--c5-batch", type=int, default=8192)
+    ap.add_argument("--c5-ema", type=float, default=0.999,
+                    help="decay of the parameters' moving average used for inference (0: off)")
     ap.add_argument("--c5-ref-spp", type=int, default=16384, help="frames of the converged plain reference")
     ap.add_argument("--c5-width", type=int, default=1920)
     ap.add_argument("--c5-height", type=int, default=1080)
@@ -423,6 +425,7 @@ def bench_c5(args, pkg, cfg):
     n_points = ctx.learn_dataset_size()
     t0 = time.perf_counter()
     ctx.mlp_init(11)
+    ctx.mlp_set_ema(args.c5_ema)
     loss = None
     n_steps = args.c5_train_steps
     for s in range(0, n_steps, 1000):
@@ -523,7 +526,8 @@ def bench_c5(args, pkg, cfg):
         "learned": {"spp": l_spp, "seconds": l_s, "relMSE": l_e,
                     "progressive_curve_spp_s_relmse": [[a, round(b, 4), round(c, 6)] for a, b, c in learned]},
         "training": {"points": n_points, "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
-                     "steps": args.c5_train_steps, "batch": args.c5_batch, "final_loss": loss},
+                     "steps": args.c5_train_steps, "batch": args.c5_batch, "ema_decay": args.c5_ema,
+                     "final_loss": loss},
         "reference_s": round(t_ref, 2),
     }
     ctx.close()

@@ -240,6 +240,9 @@ class Context:
     def mlp_init(self, seed=1):
         self.check(self.lib.cr_mlp_init(self.h, seed), "cr_mlp_init")
 
+    def mlp_set_ema(self, decay):
+        self.check(self.lib.cr_mlp_set_ema(self.h, decay), "cr_mlp_set_ema")
+
     # ------------------------------------------------------- batched render
     def batch_add(self, frames):
         arr = (structs.PushConstants * len(frames))(*frames)

@@ -91,6 +91,8 @@ SIGNATURES = {
     "cr_mlp_init": (st, [ctx_p, C.c_uint32]),
     "cr_mlp_set_params": (st, [ctx_p, vp, C.c_uint32]),
     "cr_mlp_get_params": (st, [ctx_p, vp, C.c_uint32]),
+    "cr_mlp_set_ema": (st, [ctx_p, C.c_float]),
+    "cr_mlp_get_ema": (st, [ctx_p, vp, C.c_uint32]),
     "cr_mlp_forward": (st, [ctx_p, vp, C.c_uint32, vp]),
     "cr_mlp_train_step": (st, [ctx_p, vp, vp, C.c_uint32, C.c_float, P(C.c_float), vp]),
     "cr_mlp_probe": (st, [ctx_p, vp, vp, vp]),

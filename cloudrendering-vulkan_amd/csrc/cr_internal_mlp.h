@@ -48,6 +48,12 @@ struct AdamArgs {
     float* v;
     uint32_t n;
     float lr, beta1, beta2, eps, c1, c2;
+    // moving average (null ema: off): ema = d ema + (1 - d) params; emaHat = ema * emaCorr
+    float* ema;
+    float* emaHat;
+    uint16_t* EW[MLP_LAYERS];   // bf16 copies of emaHat, [K][N] and [N][K]
+    uint16_t* EWT[MLP_LAYERS];
+    float emaDecay, emaCorr;
 };
 
 hipError_t launch_mlp_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);

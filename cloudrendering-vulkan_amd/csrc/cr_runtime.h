@@ -95,6 +95,8 @@ cr_status fold_pt_events(cr_context* ctx);
 cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb);
 void ppb_free(cr_context* ctx);
 void mlp_free(cr_context* ctx);
+// Parameters have been set (cr_mlp_init / cr_mlp_set_params).
+bool mlp_initialized(cr_context* ctx);
 void learn_free(cr_context* ctx);
 void batch_free(cr_context* ctx);
 cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr);

@@ -401,7 +401,7 @@ cr_status cr_learn_train(cr_context* ctx, uint32_t steps, uint32_t batch, float 
     hipSetDevice(ctx->device);
     Learn* L = learn_get(ctx);
     if (L->dsN == 0) return fail(ctx, CR_ERR_STATE, "empty training data set (cr_learn_collect)");
-    if (!ctx->mlp) CR_TRY(cr_mlp_init(ctx, seed));
+    if (!mlp_initialized(ctx)) CR_TRY(cr_mlp_init(ctx, seed));
     CR_TRY(ensure_buffer(ctx, (void**)&L->bX, &L->bXCap, (size_t)batch * 448, "batch X"));
     CR_TRY(ensure_buffer(ctx, (void**)&L->bT, &L->bTCap, (size_t)batch * 12, "batch T"));
     for (uint32_t s = 0; s < steps; s++) {
@@ -419,7 +419,7 @@ cr_status cr_learn_render(cr_context* ctx, const cr_push_constants* frames, uint
     if (!ctx) return CR_ERR_INVALID_ARGUMENT;
     if (!frames && n_frames) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null frames");
     if (n_frames == 0) return CR_OK;
-    if (!ctx->mlp) return fail(ctx, CR_ERR_STATE, "no network (cr_mlp_init / cr_learn_train)");
+    if (!mlp_initialized(ctx)) return fail(ctx, CR_ERR_STATE, "no network (cr_mlp_init / cr_learn_train)");
     hipSetDevice(ctx->device);
     Learn* L = learn_get(ctx);
     CR_TRY(check_render_state(ctx));

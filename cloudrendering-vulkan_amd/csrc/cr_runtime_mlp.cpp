@@ -22,6 +22,7 @@ struct Mlp {
     uint16_t* W[MLP_LAYERS] = {};
     uint16_t* WT[MLP_LAYERS] = {};
     uint64_t step = 0;
+    bool initialized = false;  // parameters set (cr_mlp_init / cr_mlp_set_params)
     // scratch of the last batch size
     uint16_t* X = nullptr;
     size_t XCap = 0;
@@ -38,6 +39,13 @@ struct Mlp {
     unsigned lossBlocks = 0;
     float* partial = nullptr;  // per-wave gradient sums of the backward pass
     size_t partialCap = 0;
+    // moving average of the parameters (allocated when first turned on)
+    float emaDecay = 0.0f;     // 0: off
+    uint64_t emaSteps = 0;     // steps averaged since the last restart
+    float* ema = nullptr;
+    float* emaHat = nullptr;   // bias-corrected average, used for inference
+    uint16_t* EW[MLP_LAYERS] = {};
+    uint16_t* EWT[MLP_LAYERS] = {};
 };
 
 namespace {
@@ -132,6 +140,15 @@ cr_status ensure_batch(cr_context* ctx, Mlp* m, uint64_t B, bool train)
     return st;
 }
 
+// Restart the moving average (cr_mlp_init, cr_mlp_set_params, cr_mlp_set_ema).
+cr_status ema_restart(cr_context* ctx, Mlp* m)
+{
+    m->emaSteps = 0;
+    if (!m->ema) return CR_OK;
+    hipError_t e = zero_async(m->ema, (size_t)m->total * 4, ctx->stream);
+    return e == hipSuccess ? CR_OK : hip_fail(ctx, e, "zero ema");
+}
+
 MlpArgs args_of(Mlp* m, const uint16_t* X, uint64_t B)
 {
     MlpArgs a;
@@ -156,19 +173,37 @@ MlpArgs args_of(Mlp* m, const uint16_t* X, uint64_t B)
     return a;
 }
 
+// Forward arguments for inference: the moving average once it holds a step, else the parameters.
+MlpArgs inference_args(Mlp* m, const uint16_t* X, uint64_t B)
+{
+    MlpArgs a = args_of(m, X, B);
+    if (m->emaDecay > 0.0f && m->emaSteps > 0) {
+        for (int l = 0; l < MLP_LAYERS; l++) {
+            a.WT[l] = m->EWT[l];
+            a.W[l] = m->EW[l];
+            a.bias[l] = m->emaHat + m->bOff[l];
+        }
+    }
+    return a;
+}
+
 } // namespace
 
 namespace cr {
+bool mlp_initialized(cr_context* ctx) { return ctx->mlp && ctx->mlp->initialized; }
+
 void mlp_free(cr_context* ctx)
 {
     Mlp* m = ctx->mlp;
     if (!m) return;
-    void* bufs[] = {m->params, m->grad, m->m, m->v, m->X, m->T, m->Y, m->dY, m->acts, m->loss, m->partial};
+    void* bufs[] = {m->params, m->grad, m->m, m->v, m->X, m->T, m->Y, m->dY, m->acts, m->loss, m->partial, m->ema, m->emaHat};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (int l = 0; l < MLP_LAYERS; l++) {
         if (m->W[l]) hipFree(m->W[l]);
         if (m->WT[l]) hipFree(m->WT[l]);
+        if (m->EW[l]) hipFree(m->EW[l]);
+        if (m->EWT[l]) hipFree(m->EWT[l]);
     }
     delete m;
     ctx->mlp = nullptr;
@@ -180,7 +215,7 @@ cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, flo
     Mlp* m;
     cr_status st = mlp_get(ctx, &m);
     if (st != CR_OK) return st;
-    MlpArgs a = args_of(m, X, B);
+    MlpArgs a = inference_args(m, X, B);
     a.Y = Y;
     a.Bdev = Bdev;
     hipError_t e = launch_mlp_forward(a, false, ctx->stream);
@@ -238,8 +273,10 @@ cr_status cr_mlp_init(cr_context* ctx, uint32_t seed)
     CR_TRY(upload_params(ctx, m, p.data()));
     CR_HIP(ctx, zero_async(m->m, (size_t)m->total * 4, ctx->stream), "zero m");
     CR_HIP(ctx, zero_async(m->v, (size_t)m->total * 4, ctx->stream), "zero v");
+    CR_TRY(ema_restart(ctx, m));
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     m->step = 0;
+    m->initialized = true;
     return CR_OK;
 }
 
@@ -253,8 +290,48 @@ cr_status cr_mlp_set_params(cr_context* ctx, const float* params, uint32_t n)
     CR_TRY(upload_params(ctx, m, params));
     CR_HIP(ctx, zero_async(m->m, (size_t)m->total * 4, ctx->stream), "zero m");
     CR_HIP(ctx, zero_async(m->v, (size_t)m->total * 4, ctx->stream), "zero v");
+    CR_TRY(ema_restart(ctx, m));
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     m->step = 0;
+    m->initialized = true;
+    return CR_OK;
+}
+
+cr_status cr_mlp_set_ema(cr_context* ctx, float decay)
+{
+    CR_CHECK_CTX(ctx);
+    if (!(decay >= 0.0f && decay < 1.0f)) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "decay must be in [0, 1)");
+    hipSetDevice(ctx->device);
+    Mlp* m;
+    CR_TRY(mlp_get(ctx, &m));
+    if (decay > 0.0f && !m->ema) {
+        size_t cap = 0;
+        CR_TRY(ensure_buffer(ctx, (void**)&m->ema, &cap, (size_t)m->total * 4, "mlp ema"));
+        cap = 0;
+        CR_TRY(ensure_buffer(ctx, (void**)&m->emaHat, &cap, (size_t)m->total * 4, "mlp ema (corrected)"));
+        for (int l = 0; l < MLP_LAYERS; l++) {
+            cap = 0;
+            CR_TRY(ensure_buffer(ctx, (void**)&m->EW[l], &cap, (size_t)MLP_K[l] * MLP_N[l] * 2, "mlp ema W"));
+            cap = 0;
+            CR_TRY(ensure_buffer(ctx, (void**)&m->EWT[l], &cap, (size_t)MLP_K[l] * MLP_N[l] * 2, "mlp ema WT"));
+        }
+    }
+    m->emaDecay = decay;
+    CR_TRY(ema_restart(ctx, m));
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    return CR_OK;
+}
+
+cr_status cr_mlp_get_ema(cr_context* ctx, float* params, uint32_t n)
+{
+    CR_CHECK_CTX(ctx);
+    if (!params) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null params");
+    Mlp* m;
+    CR_TRY(mlp_get(ctx, &m));
+    if (n != m->total) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "parameter count must equal the padded layout total");
+    if (!(m->emaDecay > 0.0f) || m->emaSteps == 0) return fail(ctx, CR_ERR_STATE, "no moving average (cr_mlp_set_ema, then train)");
+    CR_HIP(ctx, hipMemcpyAsync(params, m->emaHat, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy ema");
+    CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     return CR_OK;
 }
 
@@ -264,6 +341,7 @@ cr_status cr_mlp_get_params(cr_context* ctx, float* params, uint32_t n)
     if (!params) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null params");
     Mlp* m;
     CR_TRY(mlp_get(ctx, &m));
+    if (!m->initialized) return fail(ctx, CR_ERR_STATE, "no network parameters (cr_mlp_init / cr_mlp_set_params)");
     if (n != m->total) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "parameter count must equal the padded layout total");
     CR_HIP(ctx, hipMemcpyAsync(params, m->params, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "copy params");
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
@@ -278,9 +356,10 @@ cr_status cr_mlp_forward(cr_context* ctx, const uint16_t* X, uint32_t B, float* 
     hipSetDevice(ctx->device);
     Mlp* m;
     CR_TRY(mlp_get(ctx, &m));
+    if (!m->initialized) return fail(ctx, CR_ERR_STATE, "no network parameters (cr_mlp_init / cr_mlp_set_params)");
     CR_TRY(ensure_batch(ctx, m, B, false));
     CR_HIP(ctx, hipMemcpyAsync(m->X, X, (size_t)B * 224 * 2, hipMemcpyHostToDevice, ctx->stream), "copy X");
-    MlpArgs a = args_of(m, m->X, B);
+    MlpArgs a = inference_args(m, m->X, B);
     CR_HIP(ctx, launch_mlp_forward(a, false, ctx->stream), "mlp forward");
     std::vector<float> y((size_t)B * 4);
     CR_HIP(ctx, hipMemcpyAsync(y.data(), m->Y, y.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "copy Y");
@@ -299,6 +378,7 @@ cr_status cr_mlp_train_step(cr_context* ctx, const uint16_t* X, const float* tar
     hipSetDevice(ctx->device);
     Mlp* m;
     CR_TRY(mlp_get(ctx, &m));
+    if (!m->initialized) return fail(ctx, CR_ERR_STATE, "no network parameters (cr_mlp_init / cr_mlp_set_params)");
     CR_TRY(ensure_batch(ctx, m, B, true));
     hipStream_t s = ctx->stream;
     CR_HIP(ctx, hipMemcpyAsync(m->X, X, (size_t)B * 224 * 2, hipMemcpyHostToDevice, s), "copy X");
@@ -375,6 +455,25 @@ cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, u
     o.eps = 1e-8f;
     o.c1 = (float)(1.0 - std::pow(0.9, (double)m->step));
     o.c2 = (float)(1.0 - std::pow(0.999, (double)m->step));
+    o.ema = nullptr;
+    o.emaHat = nullptr;
+    o.emaDecay = 0.0f;
+    o.emaCorr = 1.0f;
+    for (int l = 0; l < MLP_LAYERS; l++) {
+        o.EW[l] = nullptr;
+        o.EWT[l] = nullptr;
+    }
+    if (m->emaDecay > 0.0f) {
+        m->emaSteps++;
+        o.ema = m->ema;
+        o.emaHat = m->emaHat;
+        o.emaDecay = m->emaDecay;
+        o.emaCorr = (float)(1.0 / (1.0 - std::pow((double)m->emaDecay, (double)m->emaSteps)));
+        for (int l = 0; l < MLP_LAYERS; l++) {
+            o.EW[l] = m->EW[l];
+            o.EWT[l] = m->EWT[l];
+        }
+    }
     CR_HIP(ctx, launch_mlp_adam(a, o, s), "mlp adam");
     return CR_OK;
 }

@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(256) mlp_bwd_kernel(cr::MlpArgs a)
 }
 
 // Gradient = the waves' slots added in order; Adam over the flat fp32
-// parameters, then refresh the bf16 copies
+// parameters, the moving average when it is on, then refresh the bf16 copies
 // (k-major W for backward, n-major WT for forward).
 __global__ void __launch_bounds__(256) mlp_adam_kernel(cr::MlpArgs a, cr::AdamArgs o)
 {
@@ -321,7 +321,15 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(cr::MlpArgs a, cr::AdamAr
     o.v[i] = v;
     const float p = o.params[i] - o.lr * fdiv(fdiv(m, o.c1), fsqrt(fdiv(v, o.c2)) + o.eps);
     o.params[i] = p;
-    // locate i in the padded layout
+    // moving average of the parameters, bias-corrected for inference
+    float eh = 0.0f;
+    if (o.ema) {
+        const float e = o.emaDecay * o.ema[i] + (1.0f - o.emaDecay) * p;
+        o.ema[i] = e;
+        eh = e * o.emaCorr;
+        o.emaHat[i] = eh;
+    }
+    // locate i in the padded layout; weights get their bf16 copies
     for (int li = 0; li < 5; li++) {
         const uint32_t K = li == 0 ? 224u : 128u, N = li < 4 ? 128u : 16u;
         const uint32_t w0 = a.wOff[li], b0 = a.bOff[li];
@@ -330,6 +338,11 @@ __global__ void __launch_bounds__(256) mlp_adam_kernel(cr::MlpArgs a, cr::AdamAr
             const uint16_t h = f2bf(p);
             a.W[li][k * N + n] = h;
             a.WT[li][n * K + k] = h;
+            if (o.ema) {
+                const uint16_t he = f2bf(eh);
+                o.EW[li][k * N + n] = he;
+                o.EWT[li][n * K + k] = he;
+            }
             return;
         }
         if (i >= b0 && i < b0 + N) return;

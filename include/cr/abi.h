@@ -369,6 +369,15 @@ CR_API cr_status cr_mlp_forward(cr_context* ctx, const uint16_t* X, uint32_t B, 
  * grads is not NULL, the parameter gradients (padded layout) of this step. */
 CR_API cr_status cr_mlp_train_step(cr_context* ctx, const uint16_t* X, const float* targets, uint32_t B, float lr,
                                    float* loss, float* grads);
+/* Exponential moving average of the parameters over training steps: after
+ * each Adam step, ema = decay * ema + (1 - decay) * params, and inference
+ * (cr_mlp_forward, cr_learn_render) uses ema / (1 - decay^steps) in place of
+ * the parameters once a step has been averaged.  decay in [0, 1); 0 (the
+ * default) turns it off.  cr_mlp_set_ema, cr_mlp_init and cr_mlp_set_params
+ * restart the average; training itself always updates the parameters. */
+CR_API cr_status cr_mlp_set_ema(cr_context* ctx, float decay);
+/* The bias-corrected average (padded layout); CR_ERR_STATE before any averaged step. */
+CR_API cr_status cr_mlp_get_ema(cr_context* ctx, float* params, uint32_t n);
 /* One 16x16x16 bf16 MFMA on row-major A (16x16) and B (16x16): D = A B. */
 CR_API cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t* B, float* D);
 

@@ -149,6 +149,48 @@ def test_training_is_bit_reproducible(ctx, pkg, M):
     assert not np.array_equal(runs[0][2], flat)
 
 
+def test_parameter_moving_average(ctx, pkg, M):
+    """cr_mlp_set_ema: after each Adam step the device holds d * ema + (1 - d)
+    * params, bias-corrected by 1 / (1 - d^t) (float32 arithmetic, checked bit
+    for bit against the same sequence on the host), and inference uses it: the
+    forward pass equals the forward pass of a network whose parameters are set
+    to the average.  Decay 0 turns it off again."""
+    ps = make_params(M, 11)
+    flat = pack(pkg, ps)
+    lib = pkg.load()
+    total = len(flat)
+    ctx.check(lib.cr_mlp_set_params(ctx.h, flat.ctypes.data, total), "set params")
+    d = 0.8
+    ctx.check(lib.cr_mlp_set_ema(ctx.h, d), "set ema")
+    B = 512
+    rng = np.random.default_rng(12)
+    Xb, _ = bf16(rng.random((B, 224)).astype(np.float32))
+    T = (rng.random((B, 3)) * 2.0).astype(np.float32)
+    loss = C.c_float()
+    e = np.zeros(total, np.float32)
+    p = np.zeros(total, np.float32)
+    for t in range(1, 4):
+        ctx.check(lib.cr_mlp_train_step(ctx.h, Xb.ctypes.data, T.ctypes.data, B, 0.02, C.byref(loss), None), "step")
+        ctx.check(lib.cr_mlp_get_params(ctx.h, p.ctypes.data, total), "get params")
+        e = np.float32(d) * e + (np.float32(1.0) - np.float32(d)) * p
+        corr = np.float32(1.0 / (1.0 - float(np.float32(d)) ** t))
+        got = np.zeros(total, np.float32)
+        ctx.check(lib.cr_mlp_get_ema(ctx.h, got.ctypes.data, total), "get ema")
+        assert np.array_equal((e * corr).view(np.uint32), got.view(np.uint32)), f"step {t}"
+    assert not np.array_equal(got, p)
+    Y_ema = np.zeros((B, 3), np.float32)
+    ctx.check(lib.cr_mlp_forward(ctx.h, Xb.ctypes.data, B, Y_ema.ctypes.data), "forward (average)")
+    ctx.check(lib.cr_mlp_set_ema(ctx.h, 0.0), "ema off")
+    Y_raw = np.zeros((B, 3), np.float32)
+    ctx.check(lib.cr_mlp_forward(ctx.h, Xb.ctypes.data, B, Y_raw.ctypes.data), "forward (parameters)")
+    ctx.check(lib.cr_mlp_set_params(ctx.h, got.ctypes.data, total), "set params to the average")
+    Y_set = np.zeros((B, 3), np.float32)
+    ctx.check(lib.cr_mlp_forward(ctx.h, Xb.ctypes.data, B, Y_set.ctypes.data), "forward (set)")
+    assert np.array_equal(Y_ema.view(np.uint32), Y_set.view(np.uint32))
+    assert not np.array_equal(Y_ema, Y_raw)
+    assert lib.cr_mlp_set_ema(ctx.h, 1.0) != 0  # decay must be < 1
+
+
 def test_adam_step_matches_oracle(ctx, pkg, M):
     ps = make_params(M, 5)
     flat = pack(pkg, ps)
