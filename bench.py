@@ -50,7 +50,7 @@ def parse():
                     help="dda: brick-majorant tracking + ratio-tracked shadow rays (the production tracker); "
                          "global: the reference's global majorant + shadow-volume NEE (c1's settings)")
     ap.add_argument("--c5-batch", type=int, default=8192)
-    ap.add_argument("--c5-ema", type=float, default=0.999,
+    ap.add_argument("--c5-ema", type=float, default=0.0,
                     help="decay of the parameters' moving average used for inference (0: off)")
     ap.add_argument("--c5-ref-spp", type=int, default=16384, help="frames of the converged plain reference")
     ap.add_argument("--c5-width", type=int, default=1920)
