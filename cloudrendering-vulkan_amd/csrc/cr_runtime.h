@@ -97,6 +97,8 @@ void ppb_free(cr_context* ctx);
 void mlp_free(cr_context* ctx);
 // Parameters have been set (cr_mlp_init / cr_mlp_set_params).
 bool mlp_initialized(cr_context* ctx);
+// New network parameters: forget which data set the network was trained on.
+void learn_network_reset(cr_context* ctx);
 void learn_free(cr_context* ctx);
 void batch_free(cr_context* ctx);
 cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr);

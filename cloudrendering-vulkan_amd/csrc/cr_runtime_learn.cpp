@@ -44,9 +44,37 @@ struct Learn {
     float* bT = nullptr;
     size_t bTCap = 0;
     uint32_t trainSteps = 0;
+    // whether the data set's targets leave out the first collision's
+    // shadow-volume term (-1: empty data set), and the same for the data the
+    // network was last trained on (-1: not trained from a data set)
+    int dsFirstNee = -1;
+    int netFirstNee = -1;
 };
 
 namespace cr {
+
+// The first collision's next-event term that the path tracer adds with the
+// current settings (on only for shadow-volume NEE and a bounce cap >= 1).
+FirstNee first_nee_of(const cr_context* ctx)
+{
+    FirstNee e;
+    memset(&e, 0, sizeof(e));
+    e.on = (ctx->ext.neeMode == CR_NEE_SHADOW_VOLUME && ctx->ext.bounceCap >= 1u) ? 1u : 0u;
+    e.sv = ctx->sv;
+    e.svx = (int32_t)ctx->svDimsBuilt[0];
+    e.svy = (int32_t)ctx->svDimsBuilt[1];
+    e.svz = (int32_t)ctx->svDimsBuilt[2];
+    e.svp = ctx->svp;
+    e.params = ctx->params;
+    e.intensity = ctx->params.lightIntensity;
+    e.albedo = ctx->ext.albedo;
+    return e;
+}
+
+void learn_network_reset(cr_context* ctx)
+{
+    if (ctx->learn) ctx->learn->netFirstNee = -1;
+}
 
 void learn_free(cr_context* ctx)
 {
@@ -340,6 +368,9 @@ cr_status cr_learn_collect(cr_context* ctx, const cr_push_constants* frames, uin
     Learn* L = learn_get(ctx);
     CR_TRY(check_render_state(ctx));
     CR_TRY(ensure_mips(ctx, L));
+    const FirstNee nee = first_nee_of(ctx);
+    if (L->dsN > 0 && L->dsFirstNee != (int)nee.on)
+        return fail(ctx, CR_ERR_STATE, "the data set holds targets of the other NEE mode (shadow volume vs not); clear it first");
     uint64_t n = 0;
     uint32_t m = 0;
     CR_TRY(record_pass(ctx, L, frames, n_frames, &n));
@@ -358,10 +389,11 @@ cr_status cr_learn_collect(cr_context* ctx, const cr_push_constants* frames, uin
     CR_TRY(continue_pass(ctx, sel, count, spp, L->targets));
     CR_TRY(learn_features_device(ctx, sel, count, L->X));
     CR_TRY(ensure_dataset(ctx, L, L->dsN + count));
-    CR_HIP(ctx, launch_append(L->X, reinterpret_cast<const float4*>(L->targets), 1, count, L->dsX, L->dsT, L->dsN,
-                              ctx->stream), "append");
+    CR_HIP(ctx, launch_append(L->X, reinterpret_cast<const float4*>(L->targets), sel, nee, 1, count, L->dsX, L->dsT,
+                              L->dsN, ctx->stream), "append");
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     L->dsN += count;
+    L->dsFirstNee = (int)nee.on;
     if (added) *added = count;
     return CR_OK;
 }
@@ -377,7 +409,10 @@ cr_status cr_learn_dataset_size(cr_context* ctx, uint64_t* n)
 cr_status cr_learn_dataset_clear(cr_context* ctx)
 {
     if (!ctx) return CR_ERR_INVALID_ARGUMENT;
-    if (ctx->learn) ctx->learn->dsN = 0;
+    if (ctx->learn) {
+        ctx->learn->dsN = 0;
+        ctx->learn->dsFirstNee = -1;
+    }
     return CR_OK;
 }
 
@@ -410,6 +445,7 @@ cr_status cr_learn_train(cr_context* ctx, uint32_t steps, uint32_t batch, float 
         CR_TRY(mlp_train_device(ctx, L->bX, L->bT, batch, lr));
     }
     L->trainSteps += steps;
+    L->netFirstNee = L->dsFirstNee;
     if (loss) CR_TRY(mlp_last_loss(ctx, loss));
     return CR_OK;
 }
@@ -423,6 +459,9 @@ cr_status cr_learn_render(cr_context* ctx, const cr_push_constants* frames, uint
     hipSetDevice(ctx->device);
     Learn* L = learn_get(ctx);
     CR_TRY(check_render_state(ctx));
+    const FirstNee nee = first_nee_of(ctx);
+    if (L->netFirstNee >= 0 && L->netFirstNee != (int)nee.on)
+        return fail(ctx, CR_ERR_STATE, "the network was trained for the other NEE mode (shadow volume vs not)");
     CR_TRY(ensure_mips(ctx, L));
     // record -> compaction -> features -> network -> composition, all on the
     // stream: the collision count stays on the device and bounds the feature
@@ -442,6 +481,7 @@ cr_status cr_learn_render(cr_context* ctx, const cr_push_constants* frames, uint
     a.records = L->records;
     a.slot = L->slot;
     a.Y = L->Y;
+    a.nee = nee;
     CR_HIP(ctx, launch_compose(a, ctx->stream), "compose");
     return CR_OK;
 }

@@ -277,6 +277,7 @@ cr_status cr_mlp_init(cr_context* ctx, uint32_t seed)
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     m->step = 0;
     m->initialized = true;
+    learn_network_reset(ctx);
     return CR_OK;
 }
 
@@ -294,6 +295,7 @@ cr_status cr_mlp_set_params(cr_context* ctx, const float* params, uint32_t n)
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
     m->step = 0;
     m->initialized = true;
+    learn_network_reset(ctx);
     return CR_OK;
 }
 

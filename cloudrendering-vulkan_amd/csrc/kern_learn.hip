@@ -92,6 +92,19 @@ __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
 // Per pixel, frames in order: L = the escaped path's radiance, or
 // expm1(network) at its first collision, blended with the reference's running
 // mean (PathTracer.comp:420-424).
+// The first collision's shadow-volume next-event term, in the path tracer's
+// operation order (pt_kernel.h, EV_COLLIDE with thr = 1).
+__device__ __forceinline__ float first_nee(const cr::FirstNee& e, f3 pos, f3 dir)
+{
+    const f3 ldir = ld3(e.svp.lightDirection);
+    const float pdf = phase_pdf(e.params, neg(dir), ldir);
+    const float w = 1.0f * e.albedo;
+    GridView g;
+    g.data = e.sv; g.nx = e.svx; g.ny = e.svy; g.nz = e.svz; g.nbx = 0; g.nby = 0;
+    const float Tv = trilinear<float, LAYOUT_DENSE>(g, sv_norm(e.svp, pos));
+    return (((1.0f * e.intensity) * pdf) * Tv) * w;
+}
+
 __global__ void __launch_bounds__(256) compose_kernel(cr::ComposeArgs a)
 {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
@@ -103,9 +116,15 @@ __global__ void __launch_bounds__(256) compose_kernel(cr::ComposeArgs a)
         float4 L;
         if (slot != 0xFFFFFFFFu) {
             const float* y = a.Y + (size_t)slot * 4;
-            L.x = fmaxf(expm1f(y[0]), 0.0f);
-            L.y = fmaxf(expm1f(y[1]), 0.0f);
-            L.z = fmaxf(expm1f(y[2]), 0.0f);
+            float e = 0.0f;
+            if (a.nee.on) {
+                const float4* r = reinterpret_cast<const float4*>(a.records) + 2 * s;
+                const float4 r0 = r[0], r1 = r[1];
+                e = first_nee(a.nee, mk(r0.x, r0.y, r0.z), mk(r0.w, r1.x, r1.y));
+            }
+            L.x = e + fmaxf(expm1f(y[0]), 0.0f);
+            L.y = e + fmaxf(expm1f(y[1]), 0.0f);
+            L.z = e + fmaxf(expm1f(y[2]), 0.0f);
             L.w = 1.0f;
         } else {
             L = reinterpret_cast<const float4*>(a.records)[2 * s];
@@ -200,8 +219,9 @@ __global__ void __launch_bounds__(256) select_kernel(const float* src, uint32_t 
 }
 
 // Rows k*stride (k < count) of X and T4 appended to the data set at `offset`.
-__global__ void __launch_bounds__(256) append_kernel(const uint16_t* X, const float4* T4, uint32_t stride, uint32_t count,
-                                                     uint16_t* dsX, float* dsT, uint64_t offset)
+__global__ void __launch_bounds__(256) append_kernel(const uint16_t* X, const float4* T4, const float* pts, cr::FirstNee nee,
+                                                     uint32_t stride, uint32_t count, uint16_t* dsX, float* dsT,
+                                                     uint64_t offset)
 {
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     if (k >= count) return;
@@ -210,9 +230,14 @@ __global__ void __launch_bounds__(256) append_kernel(const uint16_t* X, const fl
     uint4* xd = reinterpret_cast<uint4*>(dsX + dst * 224);
     for (int q = 0; q < 28; q++) xd[q] = xs[q];
     const float4 t = T4[src];
-    dsT[dst * 3 + 0] = t.x;
-    dsT[dst * 3 + 1] = t.y;
-    dsT[dst * 3 + 2] = t.z;
+    float e = 0.0f;
+    if (nee.on) {
+        const float* p = pts + src * 8;
+        e = first_nee(nee, mk(p[0], p[1], p[2]), mk(p[3], p[4], p[5]));
+    }
+    dsT[dst * 3 + 0] = t.x - e;
+    dsT[dst * 3 + 1] = t.y - e;
+    dsT[dst * 3 + 2] = t.z - e;
 }
 
 // A training batch: row r is data-set row hash(seed, step, r) mod n.
@@ -263,11 +288,12 @@ hipError_t launch_select(const float* src, uint32_t stride, uint32_t count, floa
     return hipGetLastError();
 }
 
-hipError_t launch_append(const uint16_t* X, const float4* T4, uint32_t stride, uint32_t count, uint16_t* dsX, float* dsT,
-                         uint64_t offset, hipStream_t s)
+hipError_t launch_append(const uint16_t* X, const float4* T4, const float* pts, const FirstNee& nee, uint32_t stride,
+                         uint32_t count, uint16_t* dsX, float* dsT, uint64_t offset, hipStream_t s)
 {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(append_kernel, dim3((count + 255) / 256), dim3(256), 0, s, X, T4, stride, count, dsX, dsT, offset);
+    hipLaunchKernelGGL(append_kernel, dim3((count + 255) / 256), dim3(256), 0, s, X, T4, pts, nee, stride, count, dsX, dsT,
+                       offset);
     return hipGetLastError();
 }
 

@@ -412,7 +412,12 @@ CR_API cr_status cr_pt_continue(cr_context* ctx, const float* records, uint32_t 
 /* Training data from the current scene: record pass over `frames`, the
  * collisions compacted in record order, every k-th of them (k chosen so that
  * at most max_points remain; 0 = all) continued with spp samples (the target)
- * and featurised; appended to the library's data set.  *added = rows added. */
+ * and featurised; appended to the library's data set.  *added = rows added.
+ * With shadow-volume NEE (and a bounce cap >= 1) the target leaves out the
+ * next-event term of the collision itself, (((1 * I) * pdf) * Tv) * albedo,
+ * which is a function of the recorded position and direction; the learned
+ * render adds it exactly.  A data set holds one kind of target: collecting
+ * under the other NEE mode fails with CR_ERR_STATE until it is cleared. */
 CR_API cr_status cr_learn_collect(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames, uint32_t spp,
                                   uint32_t max_points, uint32_t* added);
 CR_API cr_status cr_learn_dataset_size(cr_context* ctx, uint64_t* n);
@@ -425,7 +430,10 @@ CR_API cr_status cr_learn_dataset_download(cr_context* ctx, uint64_t first, uint
 CR_API cr_status cr_learn_train(cr_context* ctx, uint32_t steps, uint32_t batch, float lr, uint32_t seed, float* loss);
 /* Learned render: record pass, features and network at every collision, and
  * the running mean into the accumulation image (cr_pt_render's contract, with
- * the network standing for the rest of each path). */
+ * the network standing for the rest of each path; with shadow-volume NEE the
+ * collision's own next-event term is evaluated exactly and the network stands
+ * for the rest).  CR_ERR_STATE if the network was trained (cr_learn_train) on
+ * targets of the other NEE mode. */
 CR_API cr_status cr_learn_render(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames);
 
 #ifdef __cplusplus

@@ -9,6 +9,7 @@
   a lower relMSE against a converged path-traced reference than the plain
   path tracer at the same 4 spp.
 """
+import ctypes as C
 import importlib.util
 import os
 
@@ -89,6 +90,50 @@ def test_collected_dataset_is_targets_and_features(ctx, cfg, S):
     assert m == (n + k - 1) // k
     X2, T2 = ctx.learn_dataset()
     assert np.array_equal(X2, X[::k]) and np.array_equal(T2, T[::k])
+    ctx.learn_dataset_clear()
+
+
+def test_shadow_volume_first_nee_is_exact(ctx, cfg, S, pkg):
+    """With shadow-volume NEE the first collision's next-event term is a
+    function of the recorded position and direction: the data set leaves it
+    out of the targets and the learned render adds it back exactly.  With a
+    bounce cap of 1 and a black sky a continuation is exactly that term, so
+    (2 continuations, an exact mean) every target is 0, and a network that
+    predicts 0 renders exactly the plain tracer's image (RGB)."""
+    sc = cfg.c1(width=24, height=20, size=40, sv_axis=48)
+    sc.fbm.seed = 3
+    sc.ext = cfg.render_ext(albedo=0.9, bounce_cap=1, sky=(0.0, 0.0, 0.0), nee=S.CR_NEE_SHADOW_VOLUME,
+                            tracking=S.CR_TRACK_GLOBAL_MAJORANT)
+    sc.setup(ctx)
+    ctx.learn_dataset_clear()
+    fr = cfg.frames(2)
+    n = ctx.learn_collect(fr, 2, 0)
+    rec = ctx.pt_record(fr, sc.height, sc.width).reshape(-1, 8)
+    hits = rec[rec[:, 7] > 0.5]
+    assert n == len(hits) > 0
+    cont = ctx.pt_continue(hits, 2)[:, :3]
+    assert cont.max() > 0.0
+    X, T = ctx.learn_dataset()
+    assert not T.any(), "targets must leave out the first collision's next-event term"
+    # a network that outputs 0: the learned render is the exact term alone
+    lib = pkg.load()
+    tot = C.c_uint32()
+    lib.cr_mlp_param_layout(None, None, None, None, C.byref(tot))
+    zeros = np.zeros(tot.value, np.float32)
+    ctx.check(lib.cr_mlp_set_params(ctx.h, zeros.ctypes.data, tot.value), "zero network")
+    fr = cfg.frames(3)
+    ctx.clear_image()
+    ctx.pt_render(fr)
+    plain = ctx.download_image(sc.height, sc.width)
+    ctx.clear_image()
+    ctx.learn_render(fr)
+    learned = ctx.download_image(sc.height, sc.width)
+    assert plain[..., :3].max() > 0.0
+    assert_bitexact(learned[..., :3].copy(), plain[..., :3].copy(), "learned render with a zero network")
+    # the data set keeps one kind of target
+    sc.ext.neeMode = S.CR_NEE_RATIO_TRACKING
+    ctx.set_render_ext(sc.ext)
+    assert ctx.lib.cr_learn_collect(ctx.h, (S.PushConstants * 2)(*fr[:2]), 2, 2, 0, None) != 0
     ctx.learn_dataset_clear()
 
 
