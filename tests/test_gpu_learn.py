@@ -4,6 +4,8 @@
   passes are the path tracer cut at the first collision);
 * the training data set is exactly the continuation targets and the stencil
   features of the recorded collisions, in record order;
+* with shadow-volume NEE the first collision's next-event term is left out of
+  the targets and added back exactly by the learned render (bit for bit);
 * stencil features against the NumPy oracle (bf16 tolerance);
 * the gain: a network trained on path-traced samples, rendered at 4 spp, has
   a lower relMSE against a converged path-traced reference than the plain
