@@ -57,41 +57,56 @@ __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
     const float* b0 = a.cloud.bounds[0];
     const float* b1 = a.cloud.bounds[1];
     const float sscale = fdiv(a.cloud.densityScaling, a.cloud.baseScaling);
-    uint16_t* out = a.X + i * 224;
-    uint32_t packed[112];
-    int idx = 0;
+    // the row is written in four groups of two levels (54 bf16 = 27 words each,
+    // the last with the two cosines and padding), so few values stay live
+    uint32_t* out = reinterpret_cast<uint32_t*>(a.X + i * 224);
     float h = a.h0;
-    for (int k = 0; k < cr::LEARN_LEVELS; k++) {
-        GridView g;
-        g.data = a.mips[k];
-        g.nx = a.dims[k][0]; g.ny = a.dims[k][1]; g.nz = a.dims[k][2];
-        g.nbx = 0; g.nby = 0;
-        for (int l = -1; l <= 1; l++)
-            for (int j = -1; j <= 1; j++)
-                for (int ii = -1; ii <= 1; ii++) {
-                    const float fi = (float)ii * h, fj = (float)j * h, fl = (float)l * h;
-                    const f3 p = mk(x.x + ((fi * ex.x + fj * ey.x) + fl * ez.x), x.y + ((fi * ex.y + fj * ey.y) + fl * ez.y),
-                                    x.z + ((fi * ex.z + fj * ey.z) + fl * ez.z));
-                    float d = 0.0f;
-                    if (in_box(b0, b1, p)) d = trilinear<float, LAYOUT_DENSE>(g, cloud_norm(a.cloud, p));
-                    const uint16_t v = f2bf((d * sscale) * h);
-                    if (idx & 1) packed[idx >> 1] |= (uint32_t)v << 16;
-                    else packed[idx >> 1] = v;
-                    idx++;
-                }
-        h = h * 2.0f;
-    }
-    const float c = dot(w, ez);
-    packed[108] = (uint32_t)f2bf(c) | ((uint32_t)f2bf(c * c) << 16);
-    packed[109] = packed[110] = packed[111] = 0u;
-    uint4* o4 = reinterpret_cast<uint4*>(out);
+#pragma unroll 1
+    for (int grp = 0; grp < 4; grp++) {
+        uint32_t packed[28];
+        int idx = 0;
 #pragma unroll
-    for (int q = 0; q < 28; q++) o4[q] = make_uint4(packed[4 * q], packed[4 * q + 1], packed[4 * q + 2], packed[4 * q + 3]);
+        for (int kk = 0; kk < 2; kk++) {
+            const int k = 2 * grp + kk;
+            GridView g;
+            g.data = a.mips[k];
+            g.nx = a.dims[k][0]; g.ny = a.dims[k][1]; g.nz = a.dims[k][2];
+            g.nbx = 0; g.nby = 0;
+#pragma unroll
+            for (int l = -1; l <= 1; l++)
+#pragma unroll
+                for (int j = -1; j <= 1; j++)
+#pragma unroll
+                    for (int ii = -1; ii <= 1; ii++) {
+                        const float fi = (float)ii * h, fj = (float)j * h, fl = (float)l * h;
+                        const f3 p = mk(x.x + ((fi * ex.x + fj * ey.x) + fl * ez.x),
+                                        x.y + ((fi * ex.y + fj * ey.y) + fl * ez.y),
+                                        x.z + ((fi * ex.z + fj * ey.z) + fl * ez.z));
+                        float d = 0.0f;
+                        if (in_box(b0, b1, p)) d = trilinear<float, LAYOUT_DENSE>(g, cloud_norm(a.cloud, p));
+                        const uint16_t v = f2bf((d * sscale) * h);
+                        if (idx & 1) packed[idx >> 1] |= (uint32_t)v << 16;
+                        else packed[idx >> 1] = v;
+                        idx++;
+                    }
+            h = h * 2.0f;
+        }
+        if (grp == 3) {
+            const float c = dot(w, ez);
+            packed[27] = (uint32_t)f2bf(c) | ((uint32_t)f2bf(c * c) << 16);
+        }
+        uint32_t* o = out + 27 * grp;
+#pragma unroll
+        for (int q = 0; q < 27; q++) o[q] = packed[q];
+        if (grp == 3) {
+            o[27] = packed[27];
+            o[28] = 0u;
+            o[29] = 0u;
+            o[30] = 0u;
+        }
+    }
 }
 
-// Per pixel, frames in order: L = the escaped path's radiance, or
-// expm1(network) at its first collision, blended with the reference's running
-// mean (PathTracer.comp:420-424).
 // The first collision's shadow-volume next-event term, in the path tracer's
 // operation order (pt_kernel.h, EV_COLLIDE with thr = 1).
 __device__ __forceinline__ float first_nee(const cr::FirstNee& e, f3 pos, f3 dir)
@@ -105,6 +120,10 @@ __device__ __forceinline__ float first_nee(const cr::FirstNee& e, f3 pos, f3 dir
     return (((1.0f * e.intensity) * pdf) * Tv) * w;
 }
 
+// Per pixel, frames in order: L = the escaped path's radiance, or
+// expm1(network) at its first collision (plus the exact first next-event term
+// when there is one), blended with the reference's running mean
+// (PathTracer.comp:420-424).
 __global__ void __launch_bounds__(256) compose_kernel(cr::ComposeArgs a)
 {
     const uint32_t p = blockIdx.x * 256 + threadIdx.x;
