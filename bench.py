@@ -72,7 +72,7 @@ def log(msg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the renderer runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the path would shard by image tiles with one sum-reduce of the RGBA32F image)", file=sys.stderr)
         sys.exit(2)
     import cr_import

@@ -36,7 +36,7 @@ def frames(count, first=0, ref_frame_count=False, srand_seed=0):
     """PushConstants for frames first..first+count-1.
 
     frameCount starts at 0 so the accumulation is the exact mean of the frames
-    (deviation D7); ref_frame_count=True starts at 1 as main.cpp:262 does."""
+    (the reference starts at 1 over an uncleared image); ref_frame_count=True starts at 1 as main.cpp:262 does."""
     seeds = msvc_seeds(first + count, srand_seed)[first:]
     base = 1 if ref_frame_count else 0
     return [S.PushConstants(0.0, seeds[i], base + first + i, 0.0, 0) for i in range(count)]

@@ -438,7 +438,7 @@ cr_status cr_grid_load_xyz(cr_context* ctx, const char* path, int storage_dtype)
 }
 
 // Writes file order (z fastest) so that load(save(g)) == g; the reference's
-// Grid3D::Save writes memory order (deviation D11).
+// Grid3D::Save writes memory order (so load(save(g)) == g).
 cr_status cr_grid_save_xyz(cr_context* ctx, const char* path)
 {
     CR_CHECK_CTX(ctx);

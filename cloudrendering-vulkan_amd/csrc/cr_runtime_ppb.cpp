@@ -2,7 +2,7 @@
 // RenderTechniquePPB.cpp): beam buffers, the deterministic beam producer, the
 // LBVH build and the gather.  The reference allocates beam storage for 4096
 // beams with no overflow guard (RenderTechniquePPB.h:117, PPB_PT.comp:381);
-// here capacities are explicit and exceeding one is an error (D27).
+// here capacities are explicit and exceeding one is an error.
 #include <hip/hip_runtime.h>
 
 #include <cstring>

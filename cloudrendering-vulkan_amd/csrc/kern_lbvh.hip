@@ -2,7 +2,8 @@
 // Karras hierarchy and bottom-up AABB refit — the method of the reference's
 // PPB_PT.comp:317-333 (Morton), PPB_RadixSort_{LocalSort,PrefixSum,
 // GlobalSort}.comp (sort), PPB_GenerateHierarchy.comp (hierarchy) and
-// PPB_CalculateAABB.comp (refit), with the corrections D13-D22.
+// PPB_CalculateAABB.comp (refit), with the reference's defects corrected (each
+// kernel says how).
 //
 // Radix sort: 4 passes of 8-bit digits.  Each pass is
 //   (1) tile histograms: 4096 keys per 256-thread block, per-wave LDS
@@ -174,7 +175,7 @@ __global__ void __launch_bounds__(256) scatter_kernel(const uint32_t* keys, cons
     }
 }
 
-// delta(i, j) of Karras 2012 with the index tie-break (D18, D19).
+// delta(i, j) of Karras 2012 with the index tie-break (exact integer clz; equal codes split by index).
 __device__ __forceinline__ int delta(const uint32_t* k, int32_t n, int32_t i, int32_t j)
 {
     if (j < 0 || j > n - 1) return -1;
@@ -197,7 +198,7 @@ __global__ void __launch_bounds__(256) hierarchy_kernel(const uint32_t* k, int32
         if (delta(k, n, i, i + (l + t) * d) > dmin) l += t;
     const int32_t j = i + l * d;
     const int32_t first = min(i, j), last = max(i, j);
-    // findSplit on (first, last) (D20)
+    // findSplit on (first, last)
     const int dnode = delta(k, n, first, last);
     int32_t s = first, step = last - first;
     do {
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(256) leaf_kernel(const cr_photon_beam* beams, 
     hi[3 * L + 2] = fmaxf(pa.z + e.z, pb.z + e.z);
 }
 
-// Bottom-up union (PPB_CalculateAABB.comp:100-120, corrected D22): the first
+// Bottom-up union (PPB_CalculateAABB.comp:100-120, corrected): the first
 // lane to arrive at a node stops; the second unions both children.  Two
 // launches, so no lane ever waits for one on another XCD:
 //  * refit_local_kernel: a workgroup takes REFIT_LEAVES consecutive leaves.

@@ -6,7 +6,7 @@
 // here a counting launch and a writing launch run the same path (same RNG
 // stream) and a scan between them gives every path its slots, so the beam
 // array is a pure function of the inputs.  Corrections: sun photons start on
-// the face the light enters and the face is chosen with if / else-if (D24);
+// the face the light enters and the face is chosen with if / else-if;
 // each segment keeps its offset along the parent beam in _padding_beam[0].
 //
 // Gather: one lane per pixel; the traversal stack (64 entries, the
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) trace_kernel(cr::TraceArgs a)
         uint32_t rng;
         xs_init(rng, (uint32_t)a.p.seed * id);
         const f3 L = ld3(a.p.lightDirection);
-        // generateSunRay — PPB_PT.comp:396-430 (corrected, D24)
+        // generateSunRay — PPB_PT.comp:396-430 (corrected)
         const float dx = b1[0] - b0[0], dy = b1[1] - b0[1], dz = b1[2] - b0[2];
         const float ax = (dz * dy) * fabsf(L.x), ay = (dx * dz) * fabsf(L.y), az = (dy * dx) * fabsf(L.z);
         const float total = (ax + ay) + az;

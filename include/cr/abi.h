@@ -88,7 +88,8 @@ CR_API void cr_cloud_default(cr_cloud_properties* c);
 /* UniformBuffers.h:114-127 default ShadowVolumeProperties. */
 CR_API void cr_shadow_volume_default(cr_shadow_volume_properties* sv);
 /* UniformBuffers.h:131-143 SetLightDirection then UniformBuffers.h:145-151 SetOrigin,
- * in the order main.cpp:223-236 calls them (the pair fixes deviation D9). */
+ * in the order main.cpp:223-236 calls them (SetOrigin re-centres the
+ * volume that SetLightDirection placed off centre). */
 CR_API void cr_shadow_volume_set_light(cr_shadow_volume_properties* sv,
                                        const float light_dir[3],
                                        const float lower_bound[3],
@@ -168,7 +169,7 @@ CR_API cr_status cr_set_shadow_volume(cr_context* ctx, const cr_shadow_volume_pr
 
 /* Render extensions (none exist in the reference). */
 typedef struct cr_render_ext {
-    float albedo;          /* scattering albedo; 1 = reference (deviation D1) */
+    float albedo;          /* scattering albedo; 1 = the reference (which has none) */
     uint32_t bounceCap;    /* real collisions that scatter; CR_UNBOUNDED = reference */
     uint32_t skyMode;      /* CR_SKY_* */
     float skyColor[4];     /* radiance of the constant sky */

@@ -7,9 +7,9 @@
  * and only as the checker or the CPU baseline; the product never links it.
  *
  * Each function cites the reference file:line it restates.  Where the
- * reference is defective (SURVEY deviation ledger D1-D27) the corrected
- * method is implemented and, where it matters, a flag reproduces the
- * reference.  All floating point is IEEE binary32 evaluated in a fixed order
+ * reference is defective (an unguarded sort histogram, a racy prefix sum,
+ * sun photons started on the exit face, and the like) the corrected method
+ * is implemented and, where it matters, a flag reproduces the reference.  All floating point is IEEE binary32 evaluated in a fixed order
  * with -ffp-contract=off, so the HIP kernels, which follow the same order,
  * can be compared bit for bit.
  */

@@ -8,7 +8,10 @@
  * tree (PPB_GenerateHierarchy.comp), the AABB refit (PPB_CalculateAABB.comp)
  * and the beam gather (PPB_PE.comp).  The reference's LBVH shaders are never
  * dispatched (RenderTechniquePPB.cpp:436-519) and carry defects; the method
- * is restated with the corrections D13-D27 of the deviation ledger.
+ * is restated with those defects corrected: 30-bit Morton codes, a stable
+ * LSD radix sort, an exact integer clz with an index tie-break, findSplit on
+ * (min, max), innerCount = n - 1, leaves that index the sorted beams, and
+ * unions over bounds[1].
  */
 #include <math.h>
 #include <stdint.h>
@@ -110,7 +113,7 @@ namespace {
 inline int clz32(uint32_t v) { return v ? __builtin_clz(v) : 32; }
 
 /* delta(i,j) of Karras 2012 with the index tie-break for equal keys
- * (corrections D18, D19: exact integer clz, duplicates split by index). */
+ * (corrected: exact integer clz, duplicates split by index). */
 inline int delta(const uint32_t* k, int64_t n, int64_t i, int64_t j)
 {
     if (j < 0 || j > n - 1) return -1;
@@ -130,7 +133,7 @@ void init_nodes(cr_tree_node* nodes, uint32_t n)
         nodes[i].isLeaf = (i >= n - 1) ? 1u : 0u;
     }
     for (uint32_t i = 0; i < n; i++) {
-        nodes[n - 1 + i].left = i;    /* D21: leaves index the sorted beams */
+        nodes[n - 1 + i].left = i;    /* leaves index the sorted beams */
         nodes[n - 1 + i].right = NONE;
     }
 }
@@ -170,7 +173,7 @@ extern "C" {
 /* PPB_GenerateHierarchy.comp main (:122-169), per internal node i in
  * [0, n-2]: direction, range and split of Karras 2012 "Maximizing Parallelism
  * in the Construction of BVHs, Octrees, and k-d Trees", Figure 4, with
- * findSplit called on (min, max) of the range (D20) and innerCount = n-1 (D17).
+ * findSplit called on (min, max) of the range and innerCount = n-1.
  * Node layout as the reference: internal nodes [0, n-2], leaves at
  * innerCount + i, root 0 (parent NONE). */
 void or_karras_build(const uint32_t* k, uint32_t n32, cr_tree_node* nodes)
@@ -232,7 +235,7 @@ void or_leaf_bounds(const cr_photon_beam* b, float lo[3], float hi[3])
 }
 
 /* PPB_CalculateAABB.comp main (:100-120): leaf boxes, then the union of the
- * two children (D22: max over bounds[1]); internal nodes end with
+ * two children (corrected: max over bounds[1]); internal nodes end with
  * processed = 2 (both children arrived), as the atomic rule leaves them. */
 void or_refit(const cr_photon_beam* sorted_beams, uint32_t n, cr_tree_node* nodes)
 {

@@ -2,14 +2,14 @@
  * ORACLE — test infrastructure only.
  *
  * Photon-beam producer (shaders/PPB_PT.comp) and gather (shaders/PPB_PE.comp)
- * restated on the CPU, with the corrections of the deviation ledger:
+ * restated on the CPU, with these corrections:
  *   - sun photons only, entry face chosen against the light direction and the
- *     face selected with if / else-if (D24; the reference starts photons on
+ *     face selected with if / else-if (the reference starts photons on
  *     the exit face and always overwrites the yz choice);
  *   - deterministic output order: a counting pass and a writing pass over the
  *     same RNG stream replace the atomic appends (PPB_PT.comp:340, :381);
  *   - the segment's offset along its parent beam is kept in _padding_beam[0]
- *     so the beam transmittance is looked up at the right distance (D23);
+ *     so the beam transmittance is looked up at the right distance;
  *   - the gather term follows Jarosz et al. 2011 (beam x beam, 1D kernel):
  *     K(d/r)/r * sigma_s * Phi * T_cam * T_beam * f(theta) / sin(theta),
  *     or, with refCompat, PPB_PE.comp:394-403 as written;
@@ -139,7 +139,7 @@ PathOut trace_path(const or_scene& s, const cr_beam_trace_params& P, uint32_t id
     uint32_t rng;
     xs_init(rng, (uint32_t)P.seed * id);
     f3 L = ld3(P.lightDirection);
-    /* generateSunRay — PPB_PT.comp:396-430 (corrected, D24) */
+    /* generateSunRay — PPB_PT.comp:396-430 (corrected) */
     float dx = b1[0] - b0[0], dy = b1[1] - b0[1], dz = b1[2] - b0[2];
     float ax = (dz * dy) * fabsf(L.x), ay = (dx * dz) * fabsf(L.y), az = (dy * dx) * fabsf(L.z);
     float total = (ax + ay) + az;

@@ -87,7 +87,7 @@ def test_camera_setters(pkg, S):
     lib.cr_camera_set_fov(C.byref(c), C.byref(fov))
     assert fov.value == 150.0  # clamped (UniformBuffers.h:54)
     assert abs(c.pixelSizeY - 2 * math.tan(math.radians(75)) * 50 / 1080) < 1e-5
-    # horizontal pixel size keeps a square frustum (deviation D5)
+    # horizontal pixel size keeps a square frustum (the reference does the same)
     assert abs(c.pixelSizeX - c.pixelSizeY * 540 / 960) < 1e-6 * c.pixelSizeX
     rot = (C.c_float * 2)(100.0, 30.0)
     lib.cr_camera_set_rotation(C.byref(c), rot)

@@ -339,7 +339,7 @@ def test_pt_transmittance_closed_form(O, S, pkg, cfg, tracking):
     tmin, tmax = C.c_float(), C.c_float()
     exp_sum = var_sum = 0.0
     # pixel (0,0) has id 0: its xorshift state is 0 forever, zeta = 0 and the
-    # ray always escapes (deviation D2, kept for stream compatibility)
+    # ray always escapes (kept for stream compatibility)
     assert img[0, 0, 0] == 1.0
     img[0, 0, 0] = 0.0
     for y in range(8):
