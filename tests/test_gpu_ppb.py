@@ -139,6 +139,35 @@ def test_gather_bitexact(ctx, cfg, S, O, ref_compat, jitter):
     assert st.rays >= sc.width * sc.height * 2
 
 
+def test_gather_bitexact_multi_workgroup_tree(ctx, cfg, S, O):
+    """A tree of more than 16384 beams: its refit spans several workgroups and
+    the nodes above their boundaries come from the second refit launch; the
+    gather over it equals the oracle's bit for bit."""
+    sc, cloud, _ = setup_c1(ctx, cfg, S)
+    svp = cfg.shadow_volume(cloud, tuple(sc.cam.forward), 40)
+    ctx.set_shadow_volume(svp)
+    ctx.sv_build()
+    p = cfg.beam_params(200, seed=21, radius=0.004, albedo=0.9, max_interactions=16)
+    nb, nd = ctx.trace_beams(p, 1 << 20)
+    assert nb > 2 * 16384
+    ctx.lbvh_build()
+    ctx.clear_image()
+    fr = cfg.frames(1)
+    ext = S.GatherExt(0, 0, 0.9, 0)
+    ctx.gather(fr, ext)
+    gpu = ctx.download_image(sc.height, sc.width)
+    gb, gd = ctx.download_beams()
+    keys, perm, nodes = ctx.lbvh_download()
+    sorted_beams = (S.PhotonBeam * nb)(*[gb[int(i)] for i in perm])
+    for i in range(nb):
+        sorted_beams[i].mortonCode = int(keys[i])
+    s, _, _ = oracle_scene_from_ctx(O, ctx, sc.cam, sc.params, svp, sc.ext)
+    b = O.beam_scene(s, sorted_beams, gd, nodes, nb, 0, 0, 0.9)
+    ref = O.gather_render(b, fr)
+    assert_bitexact(gpu, ref, f"gather over {nb} beams")
+    assert np.isfinite(gpu).all() and gpu[..., 0].max() > 0.0
+
+
 def test_gather_bvh_equals_brute_force(ctx, cfg, S, O):
     """The BVH sum equals the loop over every beam (up to reassociation)."""
     sc, cloud, _ = setup_c1(ctx, cfg, S)
