@@ -168,6 +168,49 @@ def test_gather_bitexact_multi_workgroup_tree(ctx, cfg, S, O):
     assert np.isfinite(gpu).all() and gpu[..., 0].max() > 0.0
 
 
+@pytest.mark.parametrize("keep", [1, 2, 3])
+def test_gather_bitexact_tiny_trees(ctx, cfg, S, O, keep):
+    """One beam (no internal node: the root box is tested directly), two and
+    three beams: the gather equals the oracle's bit for bit."""
+    sc, cloud, _ = setup_c1(ctx, cfg, S)
+    svp = cfg.shadow_volume(cloud, tuple(sc.cam.forward), 40)
+    ctx.set_shadow_volume(svp)
+    ctx.sv_build()
+    p = cfg.beam_params(40, seed=11, radius=0.05, albedo=0.9, max_interactions=16)
+    ctx.trace_beams(p, 1 << 20)
+    gb, gd = ctx.download_beams()
+    # beams that light some pixel on their own (tried longest first), so the
+    # comparison is not one of empty images
+    fr = cfg.frames(2)
+    lengths = [sum((gb[i].endPos[k] - gb[i].startPos[k]) ** 2 for k in range(3)) for i in range(len(gb))]
+    pick = []
+    for i in sorted(range(len(gb)), key=lambda i: -lengths[i])[:64]:
+        ctx.upload_beams((S.PhotonBeam * 1)(gb[i]), gd)
+        ctx.lbvh_build()
+        ctx.clear_image()
+        ctx.gather(fr[:1], S.GatherExt(0, 0, 0.9, 0))
+        if ctx.download_image(sc.height, sc.width)[..., 0].max() > 0.0:
+            pick.append(i)
+            if len(pick) == keep:
+                break
+    assert len(pick) == keep
+    ctx.upload_beams((S.PhotonBeam * keep)(*[gb[i] for i in pick]), gd)
+    ctx.lbvh_build()
+    ctx.clear_image()
+    ctx.gather(fr, S.GatherExt(0, 0, 0.9, 0))
+    gpu = ctx.download_image(sc.height, sc.width)
+    sb, sd = ctx.download_beams()
+    keys, perm, nodes = ctx.lbvh_download()
+    sorted_beams = (S.PhotonBeam * keep)(*[sb[int(i)] for i in perm])
+    for i in range(keep):
+        sorted_beams[i].mortonCode = int(keys[i])
+    s, _, _ = oracle_scene_from_ctx(O, ctx, sc.cam, sc.params, svp, sc.ext)
+    b = O.beam_scene(s, sorted_beams, sd, nodes, keep, 0, 0, 0.9)
+    ref = O.gather_render(b, fr)
+    assert_bitexact(gpu, ref, f"gather over {keep} beam(s)")
+    assert gpu[..., 0].max() > 0.0
+
+
 def test_gather_bvh_equals_brute_force(ctx, cfg, S, O):
     """The BVH sum equals the loop over every beam (up to reassociation)."""
     sc, cloud, _ = setup_c1(ctx, cfg, S)
