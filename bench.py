@@ -358,6 +358,16 @@ def bench_c3(args, pkg, cfg):
     # hierarchy ~20 B, refit 28 B read + 48 B written
     alg = 208.0 * nb
     achieved = alg / (build_ms * 1e-3) / 1e9
+    # HBM bytes of one build, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this workload
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "lbvh_c3_traffic.json")
+    if os.path.exists(tfile):
+        try:
+            tf = json.load(open(tfile))
+            if tf.get("workload") == "c3" and tf.get("beams") == nb:
+                traffic = tf.get("hbm_bytes_per_build")
+        except Exception:
+            traffic = None
     out = {
         "metric": METRIC,
         "value": round(nb / (build_ms * 1e-3) / 1e6, 2),
@@ -374,7 +384,7 @@ def bench_c3(args, pkg, cfg):
         "stages_ms": stages,
         "trace_s": round(t_trace, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": "lbvh build (5 stages)",
+                     "frac": round(achieved / 8000.0, 4), "traffic": traffic, "kernel": "lbvh build (5 stages)",
                      "algorithmic_bytes_per_build": alg},
         "parity_vs_oracle": parity,
         "cpu_baseline": cpu,
