@@ -143,6 +143,75 @@ __global__ void __launch_bounds__(256) mlp_fwd_kernel(cr::MlpArgs a)
     }
 }
 
+// Inference: as mlp_fwd_kernel<false>, but each layer's weights are copied
+// into LDS once per workgroup (rows padded by 8 bf16 against bank conflicts)
+// and the four waves read their B fragments there, instead of every wave
+// reading the whole network from L2.  The waves meet at two barriers per
+// layer, so a wave whose rows lie past B still runs (on zero rows, storing
+// nothing); only a workgroup with no row at all returns, all waves together.
+__global__ void __launch_bounds__(256) mlp_infer_kernel(cr::MlpArgs a)
+{
+    __shared__ uint16_t lds_act[4 * 32 * ROW_STRIDE];
+    __shared__ uint16_t lds_w[128 * ROW_STRIDE];
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t B = a.Bdev ? (int64_t)*a.Bdev : a.B;
+    if ((int64_t)blockIdx.x * 128 >= B) return;
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * 32;
+    uint16_t* act = lds_act + wave * 32 * ROW_STRIDE;
+    for (int idx = l; idx < 32 * 28; idx += 64) {
+        const int r = idx / 28, c = idx % 28;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (row0 + r < B) v = *reinterpret_cast<const uint4*>(a.X + (size_t)(row0 + r) * 224 + c * 8);
+        *reinterpret_cast<uint4*>(act + r * ROW_STRIDE + c * 8) = v;
+    }
+    v4f acc[2][8];
+#pragma unroll
+    for (int li = 0; li < 5; li++) {
+        const int K = li == 0 ? 224 : 128, N = li < 4 ? 128 : 16;
+        const int WS = K + 8;
+        const uint16_t* WT = a.WT[li];
+        for (int idx = threadIdx.x; idx < N * (K / 8); idx += 256) {
+            const int n = idx / (K / 8), c = idx % (K / 8);
+            *reinterpret_cast<uint4*>(lds_w + n * WS + c * 8) = *reinterpret_cast<const uint4*>(WT + (size_t)n * K + c * 8);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int nt = 0; nt < 8; nt++) acc[rt][nt] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 2
+        for (int kb = 0; kb < K / 16; kb++) {
+            const int kk = kb * 16 + 4 * (l / 16);
+            const v4s a0 = ld4(act + (0 * 16 + l % 16) * ROW_STRIDE + kk);
+            const v4s a1 = ld4(act + (1 * 16 + l % 16) * ROW_STRIDE + kk);
+#pragma unroll
+            for (int nt = 0; nt < 8; nt++) {
+                if (nt * 16 >= N) break;
+                const v4s b = ld4(lds_w + (nt * 16 + l % 16) * WS + kk);
+                acc[0][nt] = mfma(a0, b, acc[0][nt]);
+                acc[1][nt] = mfma(a1, b, acc[1][nt]);
+            }
+        }
+        const float* bias = a.bias[li];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int nt = 0; nt < 8; nt++) {
+                if (nt * 16 >= N) break;
+                const int n = nt * 16 + l % 16;
+                const float bn = bias[n];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int r = rt * 16 + 4 * (l / 16) + j;
+                    const float z = acc[rt][nt][j] + bn;
+                    if (li < 4) act[r * ROW_STRIDE + n] = f2bf(fmaxf(z, 0.0f));
+                    else if (n < 3 && row0 + r < B) a.Y[(size_t)(row0 + r) * 4 + n] = z;
+                }
+            }
+        __syncthreads();  // every wave is done with this layer's weights
+    }
+}
+
 // dY = 2 / (B * 3) (Y - log(1 + max(t, 0))); loss[block] = this block's share
 // of the mean squared difference (the host adds the blocks in order).
 __global__ void __launch_bounds__(256) mlp_loss_kernel(const float* Y, const float* T, int64_t B, float* dY, float* loss)
@@ -364,7 +433,7 @@ hipError_t launch_mlp_forward(const MlpArgs& a, bool train, hipStream_t s)
     const unsigned blocks = (unsigned)((a.B + 127) / 128);
     if (blocks == 0) return hipSuccess;
     if (train) hipLaunchKernelGGL(mlp_fwd_kernel<true>, dim3(blocks), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(mlp_fwd_kernel<false>, dim3(blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(mlp_infer_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
