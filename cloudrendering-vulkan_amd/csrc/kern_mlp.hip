@@ -13,7 +13,8 @@
 // 8-byte fragment reads do not conflict); weight fragments are read from
 // global memory in the transposed layout WT[n][k] so each lane's four k
 // values are one 8-byte load.  Waves never exchange data, so the kernel has
-// no barriers.
+// no barriers.  Inference (mlp_infer_kernel) instead copies each layer's WT
+// into LDS once per workgroup and reads the B fragments there.
 // Backward: per wave and layer, dW += A_in^T dZ (A^T fragments from LDS, the
 // dZ accumulator tiles are already in B-operand layout), db += column sums of
 // dZ, dA = dZ W^T (dZ staged through LDS into A layout, W read k-major), and
