@@ -216,6 +216,37 @@ __device__ __forceinline__ uint64_t voxel_index(const GridView& g, int32_t x, in
     return (brick << 6) + (uint64_t)((x & 3) + ((y & 3) << 2) + ((z & 3) << 4));
 }
 
+// binary32 -> binary16 bits, round to nearest, ties to even, in integer
+// arithmetic.  Handing a float expression straight to a half conversion lets
+// the compiler fold its last multiply into v_fma_mixlo_f16, which rounds the
+// exact product to half once, where IEEE semantics first round it to
+// binary32; the two differ whenever the binary32 value is a tie between two
+// halves.
+__device__ __forceinline__ uint16_t f32_to_f16_rne(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    const uint32_t sign = (b >> 16) & 0x8000u;
+    const uint32_t a = b & 0x7fffffffu;
+    if (a >= 0x7f800000u) return (uint16_t)(sign | (a > 0x7f800000u ? 0x7e00u : 0x7c00u));
+    if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);   // >= 65520 rounds to infinity
+    uint32_t h, rem, half;
+    if (a < 0x38800000u) {                                      // below 2^-14: a subnormal half or zero
+        const uint32_t e = a >> 23;
+        if (e < 102u) return (uint16_t)sign;                    // below 2^-25 (a tie at 2^-25 goes to 0)
+        const uint32_t m = (a & 0x7fffffu) | 0x800000u;         // value = m 2^(e-150) = (m >> (126-e)) 2^-24 + rest
+        const uint32_t sh = 126u - e;                           // 14..24
+        h = m >> sh;
+        rem = m & ((1u << sh) - 1u);
+        half = 1u << (sh - 1u);
+    } else {
+        h = (((a >> 23) - 112u) << 10) | ((a & 0x7fffffu) >> 13);
+        rem = a & 0x1fffu;
+        half = 0x1000u;
+    }
+    if (rem > half || (rem == half && (h & 1u))) h += 1u;       // a carry moves into the exponent correctly
+    return (uint16_t)(sign | h);
+}
+
 template <typename T> __device__ __forceinline__ float load_voxel(const T* p, uint64_t i);
 template <> __device__ __forceinline__ float load_voxel<float>(const float* p, uint64_t i) { return p[i]; }
 template <> __device__ __forceinline__ float load_voxel<__half>(const __half* p, uint64_t i)

@@ -106,7 +106,10 @@ __device__ __forceinline__ uint64_t vidx(int32_t x, int32_t y, int32_t z, int32_
 
 template <typename T> __device__ __forceinline__ void store_voxel(T* p, uint64_t i, float v);
 template <> __device__ __forceinline__ void store_voxel<float>(float* p, uint64_t i, float v) { p[i] = v; }
-template <> __device__ __forceinline__ void store_voxel<__half>(__half* p, uint64_t i, float v) { p[i] = __float2half_rn(v); }
+template <> __device__ __forceinline__ void store_voxel<__half>(__half* p, uint64_t i, float v)
+{
+    reinterpret_cast<uint16_t*>(p)[i] = f32_to_f16_rne(v);
+}
 
 // One thread per voxel of an x-fastest 3-D launch (x in blockIdx.x*256 + tid).
 template <typename T, int LAYOUT>

@@ -154,6 +154,108 @@ def test_pt_extensions_bitexact(ctx, cfg, S, O, variant):
     assert np.isfinite(gpu).all()
 
 
+def test_fbm_anvil_bitexact(ctx, cfg, S, O):
+    """The production grid generator (8-octave anvil profile, FP16 in 4^3
+    bricks, 16^3 brick majorants: the 1024x512x1024 workload's settings) at
+    96x48x96 equals the oracle's generator voxel for voxel, and so do its brick
+    majorants and skip grid."""
+    sc = cfg.c2(width=32, height=18, size=(96, 48, 96))
+    assert sc.fbm.shape == S.CR_SHAPE_ANVIL and sc.fbm.octaves == 8
+    sc.setup(ctx, build_sv=False)
+    gpu = ctx.download_grid()
+    ref = O.fbm_grid(sc.fbm).astype(np.float16).astype(np.float32)
+    assert_bitexact(gpu, ref, "anvil fBm grid (FP16, 4^3 bricks)")
+    assert 0.0 < float(ref.mean()) < 0.5 and ref.max() == 1.0
+    maj, _ = O.brick_majorants(ref, 16)
+    assert_bitexact(ctx.majorants(), maj, "16^3 brick majorants")
+    cl = ctx.cloud()
+    assert_bitexact(ctx.skip_grid(), skip_reference(maj, cl.densityScaling, cl.baseScaling), "skip grid")
+
+
+def test_fbm_anvil_f32_bitexact(ctx, cfg, S, O):
+    """The anvil generator's float32 values, before any FP16 rounding."""
+    sc = cfg.c2(width=32, height=18, size=(96, 48, 96), storage=S.CR_DTYPE_F32, layout=S.CR_GRID_DENSE)
+    sc.setup(ctx, build_sv=False)
+    assert_bitexact(ctx.download_grid(), O.fbm_grid(sc.fbm), "anvil fBm grid (FP32)")
+
+
+def test_f16_storage_rounds_ties_to_even(ctx, S):
+    """FP16 storage rounds to nearest, ties to even: every value halfway
+    between two consecutive positive FP16 numbers (subnormal and normal), and
+    its float32 neighbours, against numpy's IEEE conversion."""
+    h = np.arange(0, 0x7BFF, dtype=np.uint16)
+    lo = h.view(np.float16).astype(np.float64)
+    hi = (h + 1).view(np.float16).astype(np.float64)
+    mid = ((lo + hi) / 2).astype(np.float32)
+    vals = np.concatenate([mid, np.nextafter(mid, np.float32(0)), np.nextafter(mid, np.float32(1e9))])
+    n = len(vals)
+    pad = (-n) % 1024
+    vals = np.concatenate([vals, np.zeros(pad, np.float32)]).reshape(-1, 32, 32)
+    for layout in (S.CR_GRID_DENSE, S.CR_GRID_BRICK4):
+        ctx.set_grid_layout(layout)
+        ctx.upload_grid(vals, 0.01, storage=S.CR_DTYPE_F16)
+        assert_bitexact(ctx.download_grid(), vals.astype(np.float16).astype(np.float32), f"FP16 rounding, layout {layout}")
+
+
+def c2_small(cfg, w=48, h=28, size=(64, 32, 64)):
+    """The production workload's render settings (FP16 anvil in 4^3 bricks,
+    16^3 brick-majorant DDA, ratio-tracked shadow rays, albedo 0.999, <= 256
+    bounces, per-axis shadow-volume dims) on a smaller grid and image."""
+    sc = cfg.c2(width=w, height=h, size=size)
+    return sc
+
+
+def test_pt_c2_shape_bitexact(ctx, cfg, S, O):
+    """The benchmarked kernel instantiation (FP16 + 4^3 bricks + brick DDA +
+    ratio tracking) with the benchmark's albedo, bounce cap and generator,
+    against the oracle (PathTracer.comp:318-361, 366-425 plus the extensions)."""
+    sc = c2_small(cfg)
+    assert sc.storage == S.CR_DTYPE_F16 and sc.grid_layout == S.CR_GRID_BRICK4
+    assert sc.ext.trackingMode == S.CR_TRACK_BRICK_DDA and sc.ext.neeMode == S.CR_NEE_RATIO_TRACKING
+    assert sc.ext.bounceCap == 256 and abs(sc.ext.albedo - 0.999) < 1e-7
+    cloud, svp = sc.setup(ctx)
+    fr = cfg.frames(4)
+    gpu, ref, st, cnt = render_both(ctx, O, sc, svp, fr)
+    assert_bitexact(gpu, ref, "c2-shaped image")
+    assert (st.paths, st.vertices, st.lookups, st.svLookups, st.brickVisits) == \
+        (cnt.paths, cnt.vertices, cnt.lookups, cnt.svLookups, cnt.brickVisits)
+    # the medium is dense enough that paths scatter many times
+    assert cnt.vertices > 3 * cnt.paths and cnt.brickVisits > cnt.lookups
+
+
+def test_pt_c2_full_size_region_bitexact(ctx, cfg, S, O):
+    """The 1024x512x1024 FP16 workload itself: sampled voxels of the generated
+    1 GiB grid equal the oracle's generator, the 64x32x64 brick majorants equal
+    the oracle's over the downloaded grid, and a 64x48-pixel block at the
+    centre of two 1920x1080 frames is bit-exact against the oracle."""
+    import ctypes as C
+    sc = cfg.c2()
+    cloud, svp = sc.setup(ctx)
+    grid = ctx.download_grid()
+    assert grid.shape == (1024, 512, 1024)
+    rng = np.random.default_rng(12)
+    idx = rng.integers(0, (1024, 512, 1024), size=(4000, 3))
+    p = sc.fbm
+    vals = np.array([O.lib.or_fbm_voxel(C.byref(p), int(x), int(y), int(z)) for x, y, z in idx], np.float32)
+    assert_bitexact(grid[idx[:, 2], idx[:, 1], idx[:, 0]], vals.astype(np.float16).astype(np.float32),
+                    "sampled voxels of the 1 GiB anvil grid")
+    maj = ctx.majorants()
+    assert maj.shape == (64, 32, 64)
+    assert_bitexact(maj, O.brick_majorants(grid, 16)[0], "64x32x64 brick majorants")
+    ctx.clear_image()
+    fr = cfg.frames(2)
+    ctx.pt_render(fr)
+    gpu = ctx.download_image(1080, 1920)
+    g16 = grid.astype(np.float16)
+    del grid
+    s = O.scene(g16, ctx.cloud(), sc.cam, sc.params, svp, sc.ext, sv=ctx.sv_download(), majorants=maj)
+    x0, y0, w, h = 928, 516, 64, 48
+    ref, cnt = O.pt_render_region(s, fr, x0, y0, w, h)
+    assert_bitexact(gpu[y0:y0 + h, x0:x0 + w], ref, "1080p block of the 1 GiB workload")
+    assert cnt.vertices > 2 * cnt.paths
+    assert np.isfinite(gpu).all()
+
+
 def test_pt_frames_split_equals_single_launch(ctx, cfg, S):
     """n frames in one persistent launch == the same frames one launch each."""
     sc = small_c1(cfg, S, w=32, h=32)
