@@ -58,6 +58,7 @@ SIGNATURES = {
     "cr_sv_download": (st, [ctx_p, vp]),
     "cr_pt_clear": (st, [ctx_p]),
     "cr_pt_render": (st, [ctx_p, P(S.PushConstants), C.c_uint32]),
+    "cr_pt_render_static": (st, [ctx_p, P(S.PushConstants), C.c_uint32, vp]),
     "cr_pt_download": (st, [ctx_p, vp]),
     "cr_pt_upload": (st, [ctx_p, vp]),
     "cr_pt_stats_get": (st, [ctx_p, P(S.PtStats)]),

@@ -123,6 +123,10 @@ cr_status cr_create(int device, cr_context** out)
         cr_destroy(ctx);
         return CR_ERR_OUT_OF_MEMORY;
     }
+    if (static_scratch_init(ctx) != CR_OK) {
+        cr_destroy(ctx);
+        return CR_ERR_DEVICE;
+    }
     zero_async(ctx->dCounters, 16 * sizeof(unsigned long long), ctx->stream);
     hipStreamSynchronize(ctx->stream);
     *out = ctx;

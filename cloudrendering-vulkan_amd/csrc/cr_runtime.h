@@ -106,4 +106,6 @@ cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, u
 cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y, const uint32_t* Bdev = nullptr);
 cr_status mlp_last_loss(cr_context* ctx, float* loss);
 cr_status fill_pt_args(cr_context* ctx, PTArgs& a, uint32_t n_frames);
+// Resolves this device's copy of the allocation-free entry point's scratch array (cr_runtime_static.cpp).
+cr_status static_scratch_init(cr_context* ctx);
 } // namespace cr

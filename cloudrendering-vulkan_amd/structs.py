@@ -88,6 +88,8 @@ class GatherStats(C.Structure):
 
 
 CR_BATCH_MAX_JOBS = 64
+CR_PT_STATIC_MAX_FRAMES = 4096
+CR_PT_STATIC_SCRATCH_BYTES = 256 + 8 * CR_PT_STATIC_MAX_FRAMES
 CR_PPB_ALPHA = 0.8  # RenderTechniquePPB.h:115 (a float32 constant in the C header)
 CR_PPB_BEAMS_PER_PASS = 56  # RenderTechniquePPB.h:118-120
 

@@ -206,6 +206,25 @@ CR_API cr_status cr_pt_render(cr_context* ctx, const cr_push_constants* frames, 
 CR_API cr_status cr_pt_download(cr_context* ctx, float* rgba);
 CR_API cr_status cr_pt_upload(cr_context* ctx, const float* rgba);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they render (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): cr_pt_render's frames rendered
+ * into a caller-owned device image `accum` (W x H RGBA32F of the current
+ * camera, read and written as the running mean) with no allocation.  The
+ * render's scratch -- work-queue counter, statistics counters and the
+ * per-frame seed table -- is one fixed CR_PT_STATIC_SCRATCH_BYTES array per
+ * GPU, resolved at cr_create; at most CR_PT_STATIC_MAX_FRAMES frames per call
+ * (CR_ERR_CAPACITY beyond).  Tiles are claimed in raster order instead of by
+ * the previous render's cost; every pixel equals cr_pt_render's bit for bit.
+ * cr_pt_stats does not see these launches.  The array is shared by every
+ * context on a GPU: calls that overlap on one GPU race.  Blocks until the
+ * frame table is copied, then returns with the render queued on the
+ * context's stream (cr_synchronize waits for it). */
+#define CR_PT_STATIC_MAX_FRAMES 4096u
+#define CR_PT_STATIC_SCRATCH_BYTES (256u + 8u * CR_PT_STATIC_MAX_FRAMES)
+CR_API cr_status cr_pt_render_static(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames,
+                                     float* accum);
+
 typedef struct cr_pt_stats {
     uint64_t paths;      /* camera paths traced (one per pixel per frame) */
     uint64_t vertices;   /* camera-ray origins + accepted (real) collisions */
