@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--c3-gather-spp", type=int, default=16, help="frames of the c3 beam gather (0: no gather)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="leave the learned-vs-plain time-to-relMSE block (C5_QUICK) out of the default line")
     ap.add_argument("--via-batch", action="store_true",
                     help="render through the batched-launch API (one job, HIP-graph replay) instead of cr_pt_render")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pt_c2_traffic.json"))
@@ -67,6 +69,19 @@ def parse():
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# the sources that define the path-tracing kernel's code
+PT_SOURCES = ("pt_kernel.h", "cr_device.h", "cr_internal.h", "kern_pt.hip")
+
+
+def pt_sources_sha16():
+    import hashlib
+    h = hashlib.sha256()
+    for name in PT_SOURCES:
+        with open(os.path.join(ROOT, "cloudrendering-vulkan_amd", "csrc", name), "rb") as f:
+            h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
 
 
 def main():
@@ -131,14 +146,21 @@ def main():
     bytes_per_launch = (16.0 * st.lookups + 4.0 * st.brickVisits + 16.0 * st.svLookups) / max(st.launches, 1) \
         + 32.0 * W * H
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, "no traffic file"
     if os.path.exists(args.traffic_file):
         try:
             tf = json.load(open(args.traffic_file))
-            if tf.get("workload") == sc.name and tf.get("spp_per_step") == spp:
+            src = pt_sources_sha16()
+            if tf.get("workload") != sc.name or tf.get("spp_per_step") != spp:
+                traffic_note = "traffic file measured on another workload"
+            elif tf.get("kernel_sources_sha16") != src:
+                traffic_note = (f"stale: traffic measured on kernel sources {tf.get('kernel_sources_sha16')}, "
+                                f"built sources {src}")
+            else:
                 traffic = tf.get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+                traffic_note = f"rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel (sources {src})"
+        except Exception as e:  # noqa: BLE001
+            traffic_note = f"unreadable traffic file: {e}"
 
     cpu = None
     if args.via_batch:
@@ -179,6 +201,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": traffic_note,
             "kernel": "pt_kernel",
             "kernel_ms": round(kern_ms, 3),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
@@ -196,6 +219,11 @@ def main():
         },
     }
     ctx.close()
+    if args.workload == "c2" and not args.no_c5:
+        log(f"c5 (quick): {C5_QUICK}")
+        t = time.perf_counter()
+        out["extra"]["c5"] = run_c5(pkg, cfg, **C5_QUICK)
+        out["extra"]["c5"]["wall_s"] = round(time.perf_counter() - t, 1)
     print(json.dumps(out), flush=True)
 
 
@@ -394,23 +422,35 @@ def bench_c3(args, pkg, cfg):
     print(json.dumps(out), flush=True)
 
 
-def bench_c5(args, pkg, cfg):
+# The learned-vs-plain measurement carried by the default line: 400 training
+# clouds with 20 000 points of 256 continuations each, so the block takes about
+# two minutes (`--workload c5` runs the full-size set: 80 000 points of 1024
+# continuations).  Cloud count matters more than points per cloud: the
+# network's error on the held-out cloud is a generalisation error.
+C5_QUICK = dict(clouds=400, points=20000, target_spp=256, train_steps=40000, collect_frames=8, tracking="dda",
+                batch=8192, ema=0.0, ref_spp=16384, W=1920, H=1080)
+C5_REF_SEED0 = 40000  # reference frame seeds 40000, 40003, ...: above MSVC rand()'s 0..32767
+
+
+def run_c5(pkg, cfg, clouds, points, target_spp, train_steps, collect_frames, tracking, batch, ema, ref_spp,
+           W, H, plain_max_spp=4096):
     """Learned in-scattering vs the plain tracer: train on samples the plain
     tracer generates over seeded 128^3 clouds of c1 statistics, then time both
-    renderers at 1080p on a held-out cloud until relMSE <= 1e-3 against a
-    converged plain render."""
+    renderers at W x H on a held-out cloud until relMSE <= 1e-3 against a
+    converged plain render.  The converged render's frame seeds are disjoint
+    from the evaluated renders' (the seed times the pixel id starts a pixel's
+    stream, so a shared seed would repeat a reference sample exactly)."""
     import math
     import numpy as np
     ctx = pkg.Context(0)
     rng = np.random.default_rng(5)
-
     S = pkg.structs
 
     def cloud_scene(seed, light, w=256, h=256):
         sc = cfg.c1(width=w, height=h)
         sc.fbm.seed = seed
         sc.light_dir = light
-        if args.c5_tracking == "dda":
+        if tracking == "dda":
             sc.ext.trackingMode = S.CR_TRACK_BRICK_DDA
             sc.ext.neeMode = S.CR_NEE_RATIO_TRACKING
         return sc
@@ -424,10 +464,10 @@ def bench_c5(args, pkg, cfg):
     # ---- training data from the plain tracer
     t0 = time.perf_counter()
     ctx.learn_dataset_clear()
-    for k in range(args.c5_clouds):
+    for k in range(clouds):
         sc = cloud_scene(1000 + k, sun())
         sc.setup(ctx)
-        ctx.learn_collect(cfg.frames(args.c5_collect_frames, srand_seed=1 + k), args.c5_target_spp, args.c5_points)
+        ctx.learn_collect(cfg.frames(collect_frames, srand_seed=1 + k), target_spp, points)
         if (k + 1) % 20 == 0:
             log(f"c5: {k + 1} clouds collected, {ctx.learn_dataset_size()} points, {time.perf_counter() - t0:.1f}s")
     ctx.synchronize()
@@ -435,31 +475,30 @@ def bench_c5(args, pkg, cfg):
     n_points = ctx.learn_dataset_size()
     t0 = time.perf_counter()
     ctx.mlp_init(11)
-    ctx.mlp_set_ema(args.c5_ema)
+    ctx.mlp_set_ema(ema)
     loss = None
-    n_steps = args.c5_train_steps
-    for s in range(0, n_steps, 1000):
-        lr = 1e-3 if s < n_steps // 3 else (3e-4 if s < 2 * n_steps // 3 else 1e-4)
-        loss = ctx.learn_train(min(1000, n_steps - s), args.c5_batch, lr, seed=3)
-        if (s // 1000) % 5 == 4:
+    for s in range(0, train_steps, 1000):
+        lr = 1e-3 if s < train_steps // 3 else (3e-4 if s < 2 * train_steps // 3 else 1e-4)
+        loss = ctx.learn_train(min(1000, train_steps - s), batch, lr, seed=3)
+        if (s // 1000) % 10 == 9:
             log(f"c5: {s + 1000} training steps, loss {loss:.4g}")
     ctx.synchronize()
     t_train = time.perf_counter() - t0
+    log(f"c5: {n_points} points collected in {t_collect:.1f}s, {train_steps} steps trained in {t_train:.1f}s")
 
-    # ---- held-out cloud at 1080p
-    W, H = args.c5_width, args.c5_height
+    # ---- held-out cloud
     sc = cloud_scene(7, (1.0, -1.0, 0.0), W, H)
     sc.setup(ctx)
     t0 = time.perf_counter()
-    ref_frames = cfg.frames(args.c5_ref_spp, srand_seed=99)
-    for i in range(0, args.c5_ref_spp, 256):
+    ref_frames = [S.PushConstants(0.0, C5_REF_SEED0 + 3 * i, i, 0.0, 0) for i in range(ref_spp)]
+    for i in range(0, ref_spp, 256):
         ctx.pt_render(ref_frames[i:i + 256])
         ctx.synchronize()
         if (i // 256) % 16 == 15:
             log(f"c5: reference {i + 256} spp, {time.perf_counter() - t0:.1f}s")
     ref = ctx.download_image(H, W)
     t_ref = time.perf_counter() - t0
-    log(f"c5: reference {args.c5_ref_spp} spp in {t_ref:.1f}s")
+    log(f"c5: reference {ref_spp} spp in {t_ref:.1f}s")
 
     def progressive(render, max_spp, chunk):
         """(spp, seconds, relMSE) after each step of a 1, 2, 3, 4, 5, 6, 7, 9, ... (x1.2) schedule;
@@ -486,7 +525,7 @@ def bench_c5(args, pkg, cfg):
                 break  # a floor: the estimator's bias, not its noise
         return out
 
-    plain = progressive(ctx.pt_render, 4096, 256)
+    plain = progressive(ctx.pt_render, plain_max_spp, 256)
     learned = progressive(ctx.learn_render, 1024, 8)
 
     def first_below(curve):
@@ -518,46 +557,112 @@ def bench_c5(args, pkg, cfg):
     p_s, p_e = timed(ctx.pt_render, p_spp, 256)
     l_s, l_e = timed(ctx.learn_render, l_spp, 8)
     log(f"c5: plain {p_spp} spp in {p_s}s (relMSE {p_e}), learned {l_spp} spp in {l_s}s (relMSE {l_e})")
-    out = {
-        "metric": METRIC,
-        "value": round(p_s / l_s, 3) if (p_s and l_s) else None,
+    ctx.close()
+    return {
+        "speedup": round(p_s / l_s, 3) if (p_s and l_s) else None,
         "unit": "x (plain time / learned time to relMSE <= 1e-3)",
-        "n_gpus": 1,
-        "higher_is_better": True,
-        "dtype": "bf16 network, f32 tracer",
-        "data": "synthetic",
-        "config": {"workload": f"c5: learned in-scattering, {args.c5_clouds} seeded 128^3 clouds of c1 statistics, "
-                               f"{args.c5_target_spp} continuations per training point, evaluation {W}x{H} on a held-out "
-                               f"cloud against a {args.c5_ref_spp}-spp plain render",
-                   "tracking": "brick-majorant DDA + ratio-tracked shadow rays" if args.c5_tracking == "dda"
-                   else "global majorant + shadow-volume NEE (the reference's)"},
+        "workload": f"c5: learned in-scattering, {clouds} seeded 128^3 clouds of c1 statistics, {points} points and "
+                    f"{target_spp} continuations per point, evaluated at {W}x{H} on a held-out cloud against a "
+                    f"{ref_spp}-spp plain render (reference seeds disjoint from the evaluated renders')",
+        "tracking": "brick-majorant DDA + ratio-tracked shadow rays" if tracking == "dda"
+        else "global majorant + shadow-volume NEE (the reference's)",
         "plain": {"spp": p_spp, "seconds": p_s, "relMSE": p_e,
                   "progressive_curve_spp_s_relmse": [[a, round(b, 4), round(c, 6)] for a, b, c in plain]},
         "learned": {"spp": l_spp, "seconds": l_s, "relMSE": l_e,
                     "progressive_curve_spp_s_relmse": [[a, round(b, 4), round(c, 6)] for a, b, c in learned]},
-        "training": {"points": n_points, "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
-                     "steps": args.c5_train_steps, "batch": args.c5_batch, "ema_decay": args.c5_ema,
+        "training": {"clouds": clouds, "points": n_points, "collect_s": round(t_collect, 2),
+                     "train_s": round(t_train, 2), "steps": train_steps, "batch": batch, "ema_decay": ema,
                      "final_loss": loss},
         "reference_s": round(t_ref, 2),
     }
-    ctx.close()
+
+
+def bench_c5(args, pkg, cfg):
+    r = run_c5(pkg, cfg, args.c5_clouds, args.c5_points, args.c5_target_spp, args.c5_train_steps,
+               args.c5_collect_frames, args.c5_tracking, args.c5_batch, args.c5_ema, args.c5_ref_spp,
+               args.c5_width, args.c5_height)
+    out = {
+        "metric": METRIC,
+        "value": r["speedup"],
+        "unit": r["unit"],
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16 network, f32 tracer",
+        "data": "synthetic",
+        "config": {"workload": r["workload"], "tracking": r["tracking"]},
+    }
+    out.update({k: r[k] for k in ("plain", "learned", "training", "reference_s")})
     print(json.dumps(out), flush=True)
 
 
+# Host cores the GPU pool grants one GPU's job (its CPU share); the process may
+# see more in nproc or its affinity mask, but those belong to other jobs.
+HOST_CORE_SHARE = 16
+
+
+def host_cpu():
+    """The host's CPU model, core counts and the cores a CPU baseline uses."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except Exception:
+        info["affinity"] = None
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = float(q) / float(period)
+    except Exception:
+        pass
+    info["cgroup_cpus"] = quota
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    info["model"] = model
+    n = info["affinity"] or info["nproc"] or 1
+    if quota:
+        n = min(n, max(1, int(quota)))
+    info["threads"] = max(1, min(n, HOST_CORE_SHARE))
+    return info
+
+
+def native_oracle():
+    """Builds the oracle's sources with -march=native for this host (the
+    baseline's build; the shipped oracle is portable x86-64-v2)."""
+    import subprocess
+    d = os.path.join(ROOT, "oracle")
+    try:
+        r = subprocess.run(["make", "-B", "-C", d, "native"], capture_output=True, timeout=300)
+        if r.returncode == 0:
+            return os.path.join(d, "_native", "liboracle.so"), "-O3 -march=native -ffp-contract=off -fopenmp"
+        log(f"native oracle build failed: {r.stderr.decode()[-400:]}")
+    except Exception as e:  # noqa: BLE001
+        log(f"native oracle build failed: {e}")
+    return None, "-O3 -march=x86-64-v2 -ffp-contract=off -fopenmp (the native build failed)"
+
+
 def cpu_baseline(ctx, sc, svp, cfg, seconds):
-    """The oracle on the host cores over a centred pixel rectangle of the
-    same render (same pixel ids and RNG streams), sized to about `seconds`."""
+    """The oracle, built -march=native on this host and run on its cores, over
+    a centred pixel band of the same render (same pixel ids and RNG streams),
+    sized to about `seconds`."""
     import numpy as np
     import cr_import
-    O = cr_import.oracle()
-    threads = max(1, min(16, os.cpu_count() or 1))
+    lib, flags = native_oracle()
+    O = cr_import.oracle(lib)
+    cpu = host_cpu()
+    threads = cpu["threads"]
     grid = ctx.download_grid()
     if sc.storage == 1:
         grid = grid.astype(np.float16)
     s = O.scene(grid, ctx.cloud(), sc.cam, sc.params, svp, sc.ext, sv=ctx.sv_download(), majorants=ctx.majorants())
     W, H = sc.width, sc.height
-    # calibrate on one frame of the image (or a centred band of it), then time
-    # a block and frame count sized to the target
+    # calibrate on one frame of a centred band, then time a block and frame
+    # count sized to the target
     w, h = W, max(1, min(H, (1 << 21) // W))
     t = time.perf_counter()
     _, cnt = O.pt_render_region(s, cfg.frames(1), 0, (H - h) // 2, w, h, threads)
@@ -573,14 +678,19 @@ def cpu_baseline(ctx, sc, svp, cfg, seconds):
     t = time.perf_counter()
     _, cnt = O.pt_render_region(s, fr, x0, y0, w, h, threads)
     dt = time.perf_counter() - t
-    log(f"cpu baseline: {w}x{h} px x {nfr} spp in {dt:.1f}s on {threads} threads")
+    log(f"cpu baseline: {w}x{h} px x {nfr} spp in {dt:.1f}s on {threads} threads ({cpu['model']})")
     return {
         "value": round(cnt.vertices / dt / 1e6, 4),
         "unit": "Mpath-vertices/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle (oracle/liboracle.so, C++ -O3, OpenMP) rendering the centred {w}x{h} pixels of the same "
-                  f"{W}x{H} image, {nfr} spp, same pixel ids and RNG streams; {cnt.vertices} vertices in {dt:.1f}s",
+        "sample": f"oracle (oracle/*.cpp, C++ {flags}, OpenMP dynamic rows, {threads} threads) rendering the "
+                  f"centred {w}x{h} pixels of the same {W}x{H} image, {nfr} spp, same pixel ids and RNG streams; "
+                  f"{cnt.vertices} vertices in {dt:.1f}s",
+        "host": {"model": cpu["model"], "nproc": cpu["nproc"], "affinity": cpu["affinity"],
+                 "cgroup_cpus": cpu["cgroup_cpus"], "threads_used": threads,
+                 "note": f"threads = min(affinity, cgroup quota, {HOST_CORE_SHARE}): the GPU pool's host-core share "
+                         "for one GPU"},
     }
 
 

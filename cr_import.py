@@ -27,11 +27,12 @@ def configs():
     return importlib.import_module(PKG_NAME + ".configs")
 
 
-def oracle():
+def oracle(lib_path=None):
     """The CPU oracle (test infrastructure): only tests/, smoke() and the
-    bench's cpu_baseline leg call this."""
+    bench's cpu_baseline leg call this.  lib_path: another build of the same
+    sources (the bench's -march=native build)."""
     pkg = package()
     spec = importlib.util.spec_from_file_location("cr_oracle_py", os.path.join(ROOT, "oracle", "oracle_py.py"))
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
-    return mod.Oracle(pkg.structs)
+    return mod.Oracle(pkg.structs, lib_path)

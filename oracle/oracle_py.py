@@ -20,12 +20,13 @@ def build():
 
 
 class Oracle:
-    def __init__(self, S):
-        """S: the structs module of the package (ctypes mirrors of the ABI)."""
-        if not os.path.exists(LIB):
+    def __init__(self, S, lib_path=None):
+        """S: the structs module of the package (ctypes mirrors of the ABI);
+        lib_path: another build of liboracle's sources (default oracle/liboracle.so)."""
+        if lib_path is None and not os.path.exists(LIB):
             build()
         self.S = S
-        self.lib = L = C.CDLL(LIB)
+        self.lib = L = C.CDLL(lib_path or LIB)
         P = C.POINTER
         vp = C.c_void_p
 
