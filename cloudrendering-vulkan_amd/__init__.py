@@ -182,6 +182,20 @@ class Context:
         a = np.ascontiguousarray(img, dtype=np.float32)
         self.check(self.lib.cr_pt_upload(self.h, _ptr(a)), "cr_pt_upload")
 
+    def pt_progressive_reset(self, srand_seed=0, first_frame_count=1, clear_on_restart=False):
+        self.check(self.lib.cr_pt_progressive_reset(self.h, srand_seed, first_frame_count, int(clear_on_restart)),
+                   "cr_pt_progressive_reset")
+
+    def pt_progressive(self, n_frames):
+        """The reference render loop's next n_frames frames (restarting the
+        running mean after any change); returns the progressive state."""
+        st = structs.ProgressiveState()
+        self.check(self.lib.cr_pt_progressive(self.h, n_frames, C.byref(st)), "cr_pt_progressive")
+        return st
+
+    def write_pfm(self, path):
+        self.check(self.lib.cr_pt_write_pfm(self.h, str(path).encode()), "cr_pt_write_pfm")
+
     def pt_stats(self):
         s = structs.PtStats()
         self.check(self.lib.cr_pt_stats_get(self.h, C.byref(s)), "cr_pt_stats_get")
@@ -432,11 +446,14 @@ class RenderTechniquePPB(RenderTechnique):
         return self.radius
 
     def AllocateResources(self, trace_params=None, max_beams=1 << 22):
+        """RenderTechniquePPB.cpp:282-355: every beam, sort and tree buffer for
+        up to max_beams beams, allocated up front."""
         self.trace_params = trace_params
         self.max_beams = max_beams
+        self.ctx.check(self.ctx.lib.cr_ppb_allocate(self.ctx.h, max_beams), "cr_ppb_allocate")
 
     def FreeResources(self):
-        pass
+        self.ctx.check(self.ctx.lib.cr_ppb_free(self.ctx.h), "cr_ppb_free")
 
     def RecordDrawCommands(self, frames, imageIndex=0, gather_ext=None, retrace=True):
         """One frame per call, as the reference: the radius of frames[0]'s

@@ -59,6 +59,9 @@ SIGNATURES = {
     "cr_pt_clear": (st, [ctx_p]),
     "cr_pt_render": (st, [ctx_p, P(S.PushConstants), C.c_uint32]),
     "cr_pt_render_static": (st, [ctx_p, P(S.PushConstants), C.c_uint32, vp]),
+    "cr_pt_progressive_reset": (st, [ctx_p, C.c_uint32, C.c_uint32, C.c_int]),
+    "cr_pt_progressive": (st, [ctx_p, C.c_uint32, P(S.ProgressiveState)]),
+    "cr_pt_write_pfm": (st, [ctx_p, C.c_char_p]),
     "cr_pt_download": (st, [ctx_p, vp]),
     "cr_pt_upload": (st, [ctx_p, vp]),
     "cr_pt_stats_get": (st, [ctx_p, P(S.PtStats)]),
@@ -82,6 +85,8 @@ SIGNATURES = {
     "cr_beams_upload": (st, [ctx_p, vp, C.c_uint32, vp, C.c_uint32]),
     "cr_beams_count": (st, [ctx_p, P(C.c_uint32), P(C.c_uint32)]),
     "cr_beams_download": (st, [ctx_p, vp, vp]),
+    "cr_ppb_allocate": (st, [ctx_p, C.c_uint32]),
+    "cr_ppb_free": (st, [ctx_p]),
     "cr_lbvh_build": (st, [ctx_p]),
     "cr_lbvh_download": (st, [ctx_p, vp, vp, vp]),
     "cr_lbvh_timings": (st, [ctx_p, P(C.c_double)]),

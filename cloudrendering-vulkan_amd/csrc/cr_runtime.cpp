@@ -285,6 +285,7 @@ cr_status grid_finish(cr_context* ctx, double vx, double vy, double vz)
     c.bounds[1][3] = -c.bounds[0][3] + 0.0f;
     ctx->hasGrid = true;
     ctx->gridVersion++;
+    ctx->sceneVersion++;
     ctx->svValid = false;
     return CR_OK;
 }
@@ -442,7 +443,9 @@ cr_status cr_grid_load_xyz(cr_context* ctx, const char* path, int storage_dtype)
 }
 
 // Writes file order (z fastest) so that load(save(g)) == g; the reference's
-// Grid3D::Save writes memory order (so load(save(g)) == g).
+// Grid3D::Save (Grid3D.h:168-178) writes memory order (x fastest), which its
+// Load reads as z fastest, so there load(save(g)) != g for a non-cubic or
+// non-symmetric grid.
 cr_status cr_grid_save_xyz(cr_context* ctx, const char* path)
 {
     CR_CHECK_CTX(ctx);
@@ -478,6 +481,7 @@ cr_status cr_set_camera(cr_context* ctx, const cr_camera_properties* cam)
             !std::isfinite(cam->up[i]))
             return fail(ctx, CR_ERR_INVALID_ARGUMENT, "non-finite camera vector");
     ctx->cam = *cam;
+    ctx->sceneVersion++;
     int W = 2 * cam->halfWidth, H = 2 * cam->halfHeight;
     if (W != ctx->W || H != ctx->H || !ctx->image) {
         cr_status st = ensure_buffer(ctx, (void**)&ctx->image, &ctx->imageCap, (size_t)W * H * 16, "image");
@@ -500,6 +504,7 @@ cr_status cr_set_cloud(cr_context* ctx, const cr_cloud_properties* c)
     const bool rescale = ctx->hasGrid && (c->densityScaling != ctx->cloud.densityScaling ||
                                           c->baseScaling != ctx->cloud.baseScaling);
     ctx->cloud = *c;
+    ctx->sceneVersion++;
     ctx->svValid = false;
     if (rescale) {
         const DeviceGrid& g = ctx->grid;
@@ -522,6 +527,7 @@ cr_status cr_set_parameters(cr_context* ctx, const cr_parameters* p)
     CR_CHECK_CTX(ctx);
     if (!p) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null parameters");
     ctx->params = *p;
+    ctx->sceneVersion++;
     return CR_OK;
 }
 
@@ -532,6 +538,7 @@ cr_status cr_set_shadow_volume(cr_context* ctx, const cr_shadow_volume_propertie
     if (s->voxelAxisCount < 2 || s->voxelAxisCount > 2048)
         return fail(ctx, CR_ERR_INVALID_ARGUMENT, "voxelAxisCount must be in [2, 2048]");
     ctx->svp = *s;
+    ctx->sceneVersion++;
     ctx->svValid = false;
     return CR_OK;
 }
@@ -554,6 +561,7 @@ cr_status cr_set_render_ext(cr_context* ctx, const cr_render_ext* e)
     if (memcmp(e->svDims, ctx->ext.svDims, sizeof(e->svDims)) != 0) ctx->svValid = false;
     const int mb = majorant_brick(*e);
     ctx->ext = *e;
+    ctx->sceneVersion++;
     if (ctx->hasGrid && ctx->grid.mb != mb) return majorants_build(ctx, ctx->grid, mb);
     return CR_OK;
 }
@@ -607,6 +615,7 @@ cr_status cr_sv_build(cr_context* ctx)
     ctx->svMs = ms;
     ctx->svDimsBuilt[0] = d[0]; ctx->svDimsBuilt[1] = d[1]; ctx->svDimsBuilt[2] = d[2];
     ctx->svValid = true;
+    ctx->sceneVersion++;
     return CR_OK;
 }
 

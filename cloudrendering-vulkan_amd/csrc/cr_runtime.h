@@ -10,6 +10,18 @@
 
 #include "cr_internal.h"
 
+// Progressive accumulation state (cr_runtime_output.cpp).
+struct Progressive {
+    uint32_t randState = 0;         // std::srand(0) (main.cpp:949)
+    uint32_t firstFrameCount = 1;   // main.cpp:262
+    bool clearOnRestart = false;
+    bool started = false;
+    uint64_t sceneVersion = 0;      // the context's sceneVersion at the last (re)start
+    uint32_t frameCount = 1;
+    uint32_t framesRendered = 0;
+    uint32_t restarts = 0;
+};
+
 struct cr_context {
     int device = 0;
     int numCUs = 0;
@@ -70,6 +82,10 @@ struct cr_context {
     uint32_t tileN = 0, tileLog2W = 0;
     bool tileCostValid = false;
     uint32_t ptTileLog2W = 3;                  // 8x8 pixel tiles
+
+    // bumped by every change to what a render reads (grid, uniforms, shadow volume)
+    uint64_t sceneVersion = 0;
+    Progressive prog;
 
     // photon beams / LBVH (cr_runtime_ppb.cpp)
     struct Ppb* ppb = nullptr;

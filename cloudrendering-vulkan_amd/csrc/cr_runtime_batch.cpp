@@ -305,6 +305,7 @@ cr_status cr_batch_add(cr_context* ctx, const cr_push_constants* frames, uint32_
     j.grid = ctx->grid;
     ctx->grid = DeviceGrid();
     ctx->hasGrid = false;
+    ctx->sceneVersion++;
     if (ctx->svValid) {
         j.sv = ctx->sv;
         memcpy(j.svDims, ctx->svDimsBuilt, sizeof(j.svDims));

@@ -85,6 +85,29 @@ static cr_status ppb_get(cr_context* ctx, Ppb** out)
 
 static const uint32_t MAX_BEAMS = 1u << 28;
 
+// Sort, tree and traversal buffers for n beams (grown, never shrunk).
+static cr_status lbvh_buffers(cr_context* ctx, Ppb* p, uint32_t n)
+{
+    const size_t nodes = 2 * (size_t)n - 1;
+    CR_TRY(ensure_buffer(ctx, (void**)&p->sorted, &p->sortedCap, (size_t)n * sizeof(cr_photon_beam), "sorted beams"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->keys, &p->keysCap, (size_t)n * 4, "keys"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->vals, &p->valsCap, (size_t)n * 4, "vals"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->keysTmp, &p->keysTmpCap, (size_t)n * 4, "keys tmp"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->valsTmp, &p->valsTmpCap, (size_t)n * 4, "vals tmp"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->counts, &p->countsCap, (size_t)sort_counts_words(n) * 4, "histograms"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->parent, &p->parentCap, nodes * 4, "parent"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->left, &p->leftCap, (size_t)n * 4, "left"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->right, &p->rightCap, (size_t)n * 4, "right"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->arrivals, &p->arrivalsCap, (size_t)n * 4, "arrivals"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->ends, &p->endsCap, (size_t)n * 4, "range ends"));
+    // refit exits (at most two per internal node) and their count at the end
+    CR_TRY(ensure_buffer(ctx, (void**)&p->exits, &p->exitsCap, (2 * (size_t)n + 1) * 4, "refit exits"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->lo, &p->loCap, nodes * 12, "box lo"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->hi, &p->hiCap, nodes * 12, "box hi"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->trav, &p->travCap, (size_t)n * sizeof(TravNode), "traversal nodes"));
+    return CR_OK;
+}
+
 extern "C" {
 
 cr_status cr_beams_upload(cr_context* ctx, const cr_photon_beam* beams, uint32_t n, const cr_photon_beam_data* data,
@@ -197,6 +220,33 @@ cr_status cr_ppb_trace_beams(cr_context* ctx, const cr_beam_trace_params* prm, u
     return CR_OK;
 }
 
+// RenderTechniquePPB::AllocateResources (RenderTechniquePPB.cpp:282-355): the
+// beam, beam-data, sort, tree and traversal buffers for up to max_beams beams
+// (and as many beam-data records), so that traces and builds of at most that
+// many beams allocate nothing more.
+cr_status cr_ppb_allocate(cr_context* ctx, uint32_t max_beams)
+{
+    CR_CHECK_CTX(ctx);
+    if (max_beams < 2 || max_beams > MAX_BEAMS) return fail(ctx, CR_ERR_CAPACITY, "max_beams must be in [2, 2^28]");
+    hipSetDevice(ctx->device);
+    Ppb* p;
+    CR_TRY(ppb_get(ctx, &p));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->beams, &p->beamsCap, (size_t)max_beams * sizeof(cr_photon_beam), "beams"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->data, &p->dataCap, (size_t)max_beams * sizeof(cr_photon_beam_data), "beam data"));
+    return lbvh_buffers(ctx, p, max_beams);
+}
+
+// RenderTechniquePPB::FreeResources: releases every beam and tree buffer (and
+// the beams themselves); the next trace or upload allocates again.
+cr_status cr_ppb_free(cr_context* ctx)
+{
+    CR_CHECK_CTX(ctx);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    ppb_free(ctx);
+    return CR_OK;
+}
+
 cr_status cr_lbvh_build(cr_context* ctx)
 {
     CR_CHECK_CTX(ctx);
@@ -205,22 +255,7 @@ cr_status cr_lbvh_build(cr_context* ctx)
     hipSetDevice(ctx->device);
     const uint32_t n = p->n;
     const size_t nodes = 2 * (size_t)n - 1;
-    CR_TRY(ensure_buffer(ctx, (void**)&p->sorted, &p->sortedCap, (size_t)n * sizeof(cr_photon_beam), "sorted beams"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->keys, &p->keysCap, (size_t)n * 4, "keys"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->vals, &p->valsCap, (size_t)n * 4, "vals"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->keysTmp, &p->keysTmpCap, (size_t)n * 4, "keys tmp"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->valsTmp, &p->valsTmpCap, (size_t)n * 4, "vals tmp"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->counts, &p->countsCap, (size_t)sort_counts_words(n) * 4, "histograms"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->parent, &p->parentCap, nodes * 4, "parent"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->left, &p->leftCap, (size_t)n * 4, "left"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->right, &p->rightCap, (size_t)n * 4, "right"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->arrivals, &p->arrivalsCap, (size_t)n * 4, "arrivals"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->ends, &p->endsCap, (size_t)n * 4, "range ends"));
-    // refit exits (at most two per internal node) and their count at the end
-    CR_TRY(ensure_buffer(ctx, (void**)&p->exits, &p->exitsCap, (2 * (size_t)n + 1) * 4, "refit exits"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->lo, &p->loCap, nodes * 12, "box lo"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->hi, &p->hiCap, nodes * 12, "box hi"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->trav, &p->travCap, (size_t)n * sizeof(TravNode), "traversal nodes"));
+    CR_TRY(lbvh_buffers(ctx, p, n));
     BoxArg box;
     for (int k = 0; k < 3; k++) {
         box.b0[k] = ctx->cloud.bounds[0][k];

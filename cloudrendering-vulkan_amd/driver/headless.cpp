@@ -3,13 +3,13 @@
 // .xyz file given on the command line, or the seeded fBm cumulus), binds the
 // uniforms in the order main.cpp's UpdateCloudData / UpdateShadowVolume use,
 // renders N frames with seeds from std::rand() after std::srand(0) and
-// writes the accumulated image as a PFM (RGB float, bottom row first).
+// writes the accumulated image as a PFM (RGB float, bottom row first) through
+// the library's progressive loop and image writer.
 //
 //   headless [frames] [width] [height] [cloud.xyz] [out.pfm]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
-#include <vector>
 
 #include "../../include/cr/techniques.hpp"
 
@@ -58,26 +58,13 @@ int main(int argc, char** argv)
         sv.RecordDrawCommands();
         cr::RenderTechniquePT pt(&dev, &cam, &push);
         pt.QueueUpdateParameters(params);
+        pt.UpdateDescriptorSets();   // commits the camera, which sizes the accumulation image
         pt.Clear();
-        std::vector<int32_t> seeds(nFrames);
-        cr_msvc_rand_sequence(0, nFrames, seeds.data());
-        std::vector<cr_push_constants> frames(nFrames);
-        for (uint32_t f = 0; f < nFrames; f++) {
-            frames[f] = push;
-            frames[f].seed = seeds[f];
-            frames[f].frameCount = f;
-        }
-        pt.RecordFrames(frames);
-        std::vector<float> img = pt.ReadImage(W, H);
-        FILE* fp = fopen(out.c_str(), "wb");
-        if (!fp) {
-            fprintf(stderr, "cannot write %s\n", out.c_str());
-            return 1;
-        }
-        fprintf(fp, "PF\n%d %d\n-1.0\n", W, H);
-        for (int y = H - 1; y >= 0; y--)
-            for (int x = 0; x < W; x++) fwrite(&img[4 * ((size_t)y * W + x)], 4, 3, fp);
-        fclose(fp);
+        // the render loop: std::rand() seeds after std::srand(0); frameCount from 0 for the exact mean
+        cr::check(dev.get(), cr_pt_progressive_reset(dev.get(), 0, 0, 1), "cr_pt_progressive_reset");
+        cr_progressive_state ps;
+        cr::check(dev.get(), cr_pt_progressive(dev.get(), nFrames, &ps), "cr_pt_progressive");
+        cr::check(dev.get(), cr_pt_write_pfm(dev.get(), out.c_str()), "cr_pt_write_pfm");
         cr_pt_stats s;
         cr_pt_stats_get(dev.get(), &s);
         printf("%u frames %dx%d: %llu paths, %llu vertices, %.3f ms -> %s\n", nFrames, W, H,

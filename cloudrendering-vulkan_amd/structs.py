@@ -94,6 +94,10 @@ CR_PPB_ALPHA = 0.8  # RenderTechniquePPB.h:115 (a float32 constant in the C head
 CR_PPB_BEAMS_PER_PASS = 56  # RenderTechniquePPB.h:118-120
 
 
+class ProgressiveState(C.Structure):
+    _fields_ = [("frameCount", u32), ("framesRendered", u32), ("restarts", u32), ("randState", u32)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("blockThreads", u32), ("blocksPerCU", u32), ("tileWidth", u32), ("_reserved", u32)]
 

@@ -225,6 +225,33 @@ CR_API cr_status cr_pt_upload(cr_context* ctx, const float* rgba);
 CR_API cr_status cr_pt_render_static(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames,
                                      float* accum);
 
+/* Progressive accumulation: the reference's render loop, headless.  Per frame
+ * the loop draws seed = std::rand() (after std::srand(0), main.cpp:722, :949)
+ * and increments frameCount (main.cpp:728); every parameter, camera or cloud
+ * change goes through UpdateShadowVolume, which restarts frameCount at 1
+ * without clearing the image (main.cpp:262, :560-590).  cr_pt_progressive
+ * renders the loop's next n_frames frames; when anything a render reads has
+ * changed since its previous call (a cr_set_* call, a grid load or
+ * generation, cr_sv_build, cr_batch_add) it first restarts frameCount at
+ * first_frame_count, clearing the image if clear_on_restart.  The defaults
+ * (srand seed 0, first_frame_count 1, no clear) reproduce the reference;
+ * first_frame_count 0 with clear_on_restart gives the exact mean of the
+ * frames since the last change.  cr_pt_progressive_reset sets them and makes
+ * the next call start afresh. */
+typedef struct cr_progressive_state {
+    uint32_t frameCount;      /* frameCount of the next frame */
+    uint32_t framesRendered;  /* frames since the last (re)start */
+    uint32_t restarts;        /* restarts caused by changes since cr_pt_progressive_reset */
+    uint32_t randState;       /* the std::rand state after the last frame's seed */
+} cr_progressive_state;
+CR_API cr_status cr_pt_progressive_reset(cr_context* ctx, uint32_t srand_seed, uint32_t first_frame_count,
+                                         int clear_on_restart);
+CR_API cr_status cr_pt_progressive(cr_context* ctx, uint32_t n_frames, cr_progressive_state* state_out);
+/* The accumulation image as a portable float map (RGB float32, little-endian,
+ * bottom row first): the headless stand-in for the swapchain blit of
+ * RenderTechniquePT::RecordDrawCommands (RenderTechniquePT.cpp:143-167). */
+CR_API cr_status cr_pt_write_pfm(cr_context* ctx, const char* path);
+
 typedef struct cr_pt_stats {
     uint64_t paths;      /* camera paths traced (one per pixel per frame) */
     uint64_t vertices;   /* camera-ray origins + accepted (real) collisions */
@@ -335,6 +362,13 @@ CR_API cr_status cr_beams_upload(cr_context* ctx, const cr_photon_beam* beams, u
                                  const cr_photon_beam_data* data, uint32_t n_data);
 CR_API cr_status cr_beams_count(cr_context* ctx, uint32_t* n, uint32_t* n_data);
 CR_API cr_status cr_beams_download(cr_context* ctx, cr_photon_beam* beams, cr_photon_beam_data* data);
+
+/* RenderTechniquePPB::AllocateResources / FreeResources (RenderTechniquePPB.h:13-16,
+ * RenderTechniquePPB.cpp:282-355): reserve every beam, sort and tree buffer for
+ * up to max_beams beams (CR_ERR_CAPACITY outside [2, 2^28]), so later traces and
+ * builds within that size allocate nothing; release them all (the beams too). */
+CR_API cr_status cr_ppb_allocate(cr_context* ctx, uint32_t max_beams);
+CR_API cr_status cr_ppb_free(cr_context* ctx);
 
 /* LBVH build over the device beams: 30-bit Morton codes of centroids in the
  * cloud box, 4 x 8-bit stable LSD radix sort, Karras hierarchy, AABB refit.

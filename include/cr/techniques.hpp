@@ -159,8 +159,13 @@ public:
         trace_.radius = initialRadius;
         cr_gather_ext_default(&gather_);
     }
-    void AllocateResources(uint32_t maxBeams = 1u << 22) { maxBeams_ = maxBeams; }
-    void FreeResources() {}
+    // RenderTechniquePPB.cpp:282-355: every beam, sort and tree buffer, up front
+    void AllocateResources(uint32_t maxBeams = 1u << 22)
+    {
+        maxBeams_ = maxBeams;
+        check(dev_->get(), cr_ppb_allocate(dev_->get(), maxBeams), "cr_ppb_allocate");
+    }
+    void FreeResources() { check(dev_->get(), cr_ppb_free(dev_->get()), "cr_ppb_free"); }
     void SetTraceParams(const cr_beam_trace_params& p) { trace_ = p; }
     void SetGatherExtensions(const cr_gather_ext& g) { gather_ = g; }
     // RenderTechniquePPB.cpp:530-546; RecordDrawCommands calls it with the frame's frameCount (:439).

@@ -252,3 +252,31 @@ def test_lbvh_full_size_bitexact(ctx, cfg, S, O):
     np.testing.assert_array_equal(perm, rp, err_msg="permutation")
     assert_same_tree(nodes, rn, nb)
     assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+
+
+def test_ppb_allocate_and_free(ctx, cfg, S, O, pkg):
+    """RenderTechniquePPB::AllocateResources / FreeResources
+    (RenderTechniquePPB.cpp:282-355): with every buffer reserved up front the
+    trace and the build give the same beams and tree as with buffers grown on
+    demand; after FreeResources the build has no beams (CR_ERR_STATE) until
+    the next trace."""
+    sc, cloud, svp = setup_c1(ctx, cfg, S)
+    p = cfg.beam_params(100, seed=38, radius=0.004, albedo=0.9, max_interactions=16)
+    tech = pkg.RenderTechniquePPB(ctx, 0.004)
+    tech.FreeResources()
+    nb, nd = ctx.trace_beams(p, 1 << 20)
+    ctx.lbvh_build()
+    k1, p1, n1 = ctx.lbvh_download()
+    tech.FreeResources()
+    with pytest.raises(pkg.CrError) as e:
+        ctx.lbvh_build()
+    assert e.value.status == S.CR_ERR_STATE
+    tech.AllocateResources(p, 1 << 20)
+    assert ctx.lib.cr_ppb_allocate(ctx.h, 1) == S.CR_ERR_CAPACITY
+    nb2, nd2 = ctx.trace_beams(p, 1 << 20)
+    ctx.lbvh_build()
+    k2, p2, n2 = ctx.lbvh_download()
+    assert (nb2, nd2) == (nb, nd)
+    np.testing.assert_array_equal(k1, k2)
+    np.testing.assert_array_equal(p1, p2)
+    assert bytes(n1) == bytes(n2)
