@@ -21,8 +21,8 @@ from . import structs
 from ._lib import LibraryNotBuilt, load  # noqa: F401
 from .structs import *  # noqa: F401,F403
 
-__all__ = ["Context", "CrError", "RenderTechniquePT", "RenderTechniqueSV", "RenderTechniquePPB", "structs", "load",
-           "LibraryNotBuilt"]
+__all__ = ["Context", "CrError", "RenderTechniquePT", "RenderTechniqueSV", "RenderTechniquePPB", "RenderTechniquePPM",
+           "structs", "load", "LibraryNotBuilt"]
 
 
 class CrError(RuntimeError):
@@ -181,6 +181,29 @@ class Context:
     def upload_image(self, img):
         a = np.ascontiguousarray(img, dtype=np.float32)
         self.check(self.lib.cr_pt_upload(self.h, _ptr(a)), "cr_pt_upload")
+
+    # ------------------------------------------------------ photon mapping
+    def set_photon_map(self, pm):
+        self.check(self.lib.cr_set_photon_map(self.h, C.byref(pm)), "cr_set_photon_map")
+
+    def ppm_trace(self, seed, params):
+        n = C.c_uint32()
+        self.check(self.lib.cr_ppm_trace(self.h, seed, C.byref(params), C.byref(n)), "cr_ppm_trace")
+        return n.value
+
+    def ppm_estimate(self, frames, params):
+        arr = (structs.PushConstants * len(frames))(*frames)
+        self.check(self.lib.cr_ppm_estimate(self.h, arr, len(frames), C.byref(params)), "cr_ppm_estimate")
+
+    def ppm_download(self):
+        """(arrivals per cell, photons sorted by cell in emission order)."""
+        nc, nd = C.c_uint32(), C.c_uint32()
+        self.check(self.lib.cr_ppm_download(self.h, C.byref(nc), C.byref(nd), None, None), "cr_ppm_download")
+        arrivals = np.empty(nc.value, np.uint32)
+        photons = (structs.Photon * max(nd.value, 1))()
+        self.check(self.lib.cr_ppm_download(self.h, C.byref(nc), C.byref(nd), _ptr(arrivals), C.addressof(photons)),
+                   "cr_ppm_download")
+        return arrivals, photons, nd.value
 
     def pt_progressive_reset(self, srand_seed=0, first_frame_count=1, clear_on_restart=False):
         self.check(self.lib.cr_pt_progressive_reset(self.h, srand_seed, first_frame_count, int(clear_on_restart)),
@@ -466,3 +489,34 @@ class RenderTechniquePPB(RenderTechnique):
             self.ctx.trace_beams(self.trace_params, self.max_beams)
         self.ctx.lbvh_build()
         self.ctx.gather(list(frames), gather_ext)
+
+
+class RenderTechniquePPM(RenderTechnique):
+    """RenderTechniquePPM (RenderTechniquePPM.cpp:229-312): per frame the
+    photon map is traced again with the frame's seed, then the estimate blends
+    the frame into the accumulation image; the radius shrinks per frame as
+    UpdateRadius does."""
+
+    def __init__(self, ctx, photon_map_properties=None, initialRadius=10.0, params=None):
+        super().__init__(ctx)
+        self.initialRadius = initialRadius
+        self.radius = initialRadius
+        self.params = params if params is not None else structs.PpmParams()
+        if params is None:
+            ctx.lib.cr_ppm_params_default(C.byref(self.params))
+        if photon_map_properties is not None:
+            ctx.set_photon_map(photon_map_properties)
+
+    def UpdateRadius(self, frameNumber):
+        self.radius = self.ctx.lib.cr_ppm_update_radius(frameNumber, structs.CR_PPM_ALPHA, self.initialRadius,
+                                                        self.radius)
+        return self.radius
+
+    def RecordDrawCommands(self, frames, imageIndex=0):
+        self.UpdateDescriptorSets()
+        if isinstance(frames, structs.PushConstants):
+            frames = [frames]
+        for f in frames:
+            f = structs.PushConstants(f.time, f.seed, f.frameCount, self.UpdateRadius(f.frameCount), f.currentBuffer)
+            self.ctx.ppm_trace(f.seed, self.params)
+            self.ctx.ppm_estimate([f], self.params)

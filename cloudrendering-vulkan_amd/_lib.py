@@ -87,6 +87,14 @@ SIGNATURES = {
     "cr_beams_download": (st, [ctx_p, vp, vp]),
     "cr_ppb_allocate": (st, [ctx_p, C.c_uint32]),
     "cr_ppb_free": (st, [ctx_p]),
+    "cr_ppm_params_default": (None, [P(S.PpmParams)]),
+    "cr_photon_map_default": (None, [P(S.PhotonMapProperties)]),
+    "cr_photon_map_set_bounds": (None, [P(S.PhotonMapProperties), P(C.c_float), P(C.c_float)]),
+    "cr_set_photon_map": (st, [ctx_p, P(S.PhotonMapProperties)]),
+    "cr_ppm_update_radius": (C.c_float, [C.c_uint32, C.c_float, C.c_float, C.c_float]),
+    "cr_ppm_trace": (st, [ctx_p, C.c_int32, P(S.PpmParams), P(C.c_uint32)]),
+    "cr_ppm_estimate": (st, [ctx_p, P(S.PushConstants), C.c_uint32, P(S.PpmParams)]),
+    "cr_ppm_download": (st, [ctx_p, P(C.c_uint32), P(C.c_uint32), vp, vp]),
     "cr_lbvh_build": (st, [ctx_p]),
     "cr_lbvh_download": (st, [ctx_p, vp, vp, vp]),
     "cr_lbvh_timings": (st, [ctx_p, P(C.c_double)]),

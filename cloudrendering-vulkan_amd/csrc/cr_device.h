@@ -168,6 +168,39 @@ __device__ __forceinline__ void crd_sincosf(float x_in, float* s_out, float* c_o
     *c_out = inDomain ? co : __builtin_nanf("");
 }
 
+// atan(t) for |t| <= tan(pi/12): its Taylor series to t^13 (truncation below
+// 3e-10 there).
+__device__ __forceinline__ float crd_atan_small(float t)
+{
+    const float z = t * t;
+    float p = 0.0769230769f;      //  1/13
+    p = p * z - 0.0909090909f;    // -1/11
+    p = p * z + 0.111111111f;     //  1/9
+    p = p * z - 0.142857143f;     // -1/7
+    p = p * z + 0.2f;             //  1/5
+    p = p * z - 0.333333333f;     // -1/3
+    return t + t * (z * p);
+}
+
+// atan2 by octant: atan of min/max in [0, 1], above tan(pi/12) reduced by
+// atan(t) = pi/6 + atan((t sqrt3 - 1) / (t + sqrt3)); then pi/2 - a, pi - a
+// and the sign of y.  atan2(0, 0) = 0.
+__device__ __forceinline__ float crd_atan2f(float y, float x)
+{
+    const float ay = fabsf(y), ax = fabsf(x);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mx == 0.0f ? 0.0f : fdiv(mn, mx);
+    const bool big = t > 0.267949194f;                       // tan(pi/12)
+    const float rt = big ? fdiv(t * 1.73205081f - 1.0f, t + 1.73205081f) : t;
+    const float at = crd_atan_small(rt);
+    float r = big ? 0.523598776f + at : at;                  // pi/6
+    r = ay > ax ? 1.57079633f - r : r;
+    r = x < 0.0f ? 3.14159265f - r : r;
+    return y < 0.0f ? -r : r;
+}
+
+__device__ __forceinline__ float crd_acosf(float x) { return crd_atan2f(fsqrt((1.0f - x) * (1.0f + x)), x); }
+
 __device__ __forceinline__ float pow15(float x) { return x * fsqrt(x); }
 
 // ------------------------------------------------------------------ RNG

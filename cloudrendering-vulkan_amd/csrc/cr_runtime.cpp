@@ -117,6 +117,7 @@ cr_status cr_create(int device, cr_context** out)
     cr_parameters_default(&ctx->params);
     cr_shadow_volume_default(&ctx->svp);
     cr_render_ext_default(&ctx->ext);
+    cr_photon_map_default(&ctx->pm);
     if (ensure_buffer(ctx, (void**)&ctx->dQueue, &ctx->queueCap, 64, "queue") != CR_OK ||
         ensure_buffer(ctx, (void**)&ctx->dCounters, &ctx->countersCap, 16 * sizeof(unsigned long long), "counters") != CR_OK ||
         ensure_buffer(ctx, (void**)&ctx->dScalar, &ctx->scalarCap, 64, "scalar") != CR_OK) {
@@ -144,6 +145,7 @@ cr_status cr_destroy(cr_context* ctx)
     for (void* b : bufs)
         if (b) hipFree(b);
     ppb_free(ctx);
+    ppm_free(ctx);
     mlp_free(ctx);
     learn_free(ctx);
     batch_free(ctx);
@@ -391,7 +393,7 @@ cr_status cr_grid_skip(cr_context* ctx, float* out)
 cr_status cr_math_probe(cr_context* ctx, int op, const float* in, uint32_t n, float* out)
 {
     CR_CHECK_CTX(ctx);
-    if (op < CR_MATH_SQRT || op > CR_MATH_COS) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "unknown math op");
+    if (op < CR_MATH_SQRT || op > CR_MATH_ATAN2) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "unknown math op");
     if (n == 0) return CR_OK;
     if (!in || !out) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null array");
     float* d = nullptr;

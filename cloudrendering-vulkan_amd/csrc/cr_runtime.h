@@ -89,6 +89,10 @@ struct cr_context {
 
     // photon beams / LBVH (cr_runtime_ppb.cpp)
     struct Ppb* ppb = nullptr;
+    // photon mapping (cr_runtime_ppm.cpp): grid properties and the photon map
+    cr_photon_map_properties pm;
+    bool hasPm = false;
+    struct Ppm* ppm = nullptr;
     // learned in-scattering network (cr_runtime_mlp.cpp) and its support data (cr_runtime_learn.cpp)
     struct Mlp* mlp = nullptr;
     struct Learn* learn = nullptr;
@@ -110,6 +114,7 @@ void fill_job_args(PTArgs& a, const DeviceGrid& g, const float* sv, const uint32
 cr_status fold_pt_events(cr_context* ctx);
 cr_status majorants_build(cr_context* ctx, DeviceGrid& g, int mb);
 void ppb_free(cr_context* ctx);
+void ppm_free(cr_context* ctx);
 void mlp_free(cr_context* ctx);
 // Parameters have been set (cr_mlp_init / cr_mlp_set_params).
 bool mlp_initialized(cr_context* ctx);

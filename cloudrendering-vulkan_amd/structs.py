@@ -46,6 +46,19 @@ class PhotonBeamData(C.Structure):  # UniformBuffers.h:190-194
     _fields_ = [("power", f32 * 4), ("trDistances", f32 * 16)]
 
 
+class Photon(C.Structure):  # UniformBuffers.h:202-210
+    _fields_ = [("position", f32 * 4), ("power", f32 * 4), ("phi", f32), ("theta", f32), ("_padding", f32 * 2)]
+
+
+class PhotonMapProperties(C.Structure):  # UniformBuffers.h:212-249
+    _fields_ = [("lightDirection", f32 * 4), ("bounds", (f32 * 4) * 2), ("voxelCount", i32 * 3), ("voxelSize", f32),
+                ("photonSize", u32), ("stepSize", f32), ("absorption", f32)]
+
+
+class PpmParams(C.Structure):
+    _fields_ = [("numPaths", u32), ("sunProbability", f32), ("ambientStep", f32), ("maxCollisions", u32)]
+
+
 class SortElement(C.Structure):  # UniformBuffers.h:196-200
     _fields_ = [("idx", u32), ("mortonCode", u32)]
 
@@ -89,6 +102,8 @@ class GatherStats(C.Structure):
 
 CR_BATCH_MAX_JOBS = 64
 CR_PT_STATIC_MAX_FRAMES = 4096
+CR_PPM_PHOTONS_PER_CELL = 32
+CR_PPM_ALPHA = 0.8  # RenderTechniquePPM.h:63 (a float32 constant in the C header)
 CR_PT_STATIC_SCRATCH_BYTES = 256 + 8 * CR_PT_STATIC_MAX_FRAMES
 CR_PPB_ALPHA = 0.8  # RenderTechniquePPB.h:115 (a float32 constant in the C header)
 CR_PPB_BEAMS_PER_PASS = 56  # RenderTechniquePPB.h:118-120

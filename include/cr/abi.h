@@ -401,6 +401,62 @@ typedef struct cr_gather_stats {
 } cr_gather_stats;
 CR_API cr_status cr_gather_stats_get(cr_context* ctx, cr_gather_stats* out);
 
+/* ---------------------------------------------------------- photon mapping
+ * RenderTechniquePPM (RenderTechniquePPM.cpp:229-312): per frame, photon
+ * tracing into a uniform grid of voxelCount cells, at most
+ * CR_PPM_PHOTONS_PER_CELL photons each (shaders/PPM_PT.comp), then a
+ * ray-marched radiance estimate per pixel: shadow-volume direct light, one
+ * sky direction, and the photons within pmRadius (shaders/PPM_PE.comp).
+ * Corrections (the estimate is otherwise the reference's):
+ *   - a cell keeps its first CR_PPM_PHOTONS_PER_CELL photons in emission
+ *     order (path id, then collision), not in atomic arrival order, so the
+ *     map is a pure function of the inputs; the estimate reads only those
+ *     (the reference loops over every arrival, past the cell's 32 slots);
+ *   - a photon stores the azimuth atan2(y, x) in [0, 2 pi) (the reference
+ *     doubles it, PPM_PT.comp:187-190, so the estimate rebuilt another
+ *     direction), and a deposit at the grid's far faces goes to the last cell;
+ *   - sun photons start on the entry faces uniformly over the projected area
+ *     (as cr_ppb_trace_beams does; the reference projects uniform points of
+ *     the volume back along the light, which its pdf 1/area does not
+ *     describe), sky photons take a uniform direction and the same area
+ *     sampling for it (the reference swaps the polar angles and offsets the
+ *     start by 0.1); the powers keep the reference's factors (1/2 per branch
+ *     probability, the two "manual adjustment" halvings, PPM_PT.comp:460-476);
+ *   - the estimate's sky ray marches with ppm.ambientStep (100 world units in
+ *     the reference, PPM_PE.comp:515) and skips the six density taps whose
+ *     mean the reference computes but never uses (PPM_PE.comp:355-366). */
+#define CR_PPM_PHOTONS_PER_CELL 32u
+#define CR_PPM_ALPHA 0.8f
+typedef struct cr_ppm_params {
+    uint32_t numPaths;        /* photon paths per frame: 200 x 64 (RenderTechniquePPM.cpp:253, PPM_PT.comp:3) */
+    float sunProbability;     /* 0.5 (PPM_PT.comp:462) */
+    float ambientStep;        /* the estimate's sky-ray step (world units) */
+    uint32_t maxCollisions;   /* ceiling on real collisions per photon path (the reference has none) */
+} cr_ppm_params;
+CR_API void cr_ppm_params_default(cr_ppm_params* p);
+/* PhotonMapProperties defaults (UniformBuffers.h:214-222) and SetBounds (:225-243). */
+CR_API void cr_photon_map_default(cr_photon_map_properties* p);
+CR_API void cr_photon_map_set_bounds(cr_photon_map_properties* p, const float b0[4], const float b1[4]);
+CR_API cr_status cr_set_photon_map(cr_context* ctx, const cr_photon_map_properties* p);
+/* RenderTechniquePPM::UpdateRadius (RenderTechniquePPM.cpp:302-312):
+ * initial_radius for frames 0 and 1, else r * ((n - 1 + alpha) / n)^(1/3). */
+CR_API float cr_ppm_update_radius(uint32_t frame_number, float alpha, float initial_radius, float previous_radius);
+/* Photon tracing with the frame seed (PPM_PT.comp) into the photon grid;
+ * deposits_out (optional): photons deposited, kept or not. */
+CR_API cr_status cr_ppm_trace(cr_context* ctx, int32_t seed, const cr_ppm_params* p, uint32_t* deposits_out);
+/* The radiance estimate (PPM_PE.comp) of n_frames frames over the current
+ * photon grid into the accumulation image (cr_pt_render's running mean);
+ * frames[f].pmRadius is the query radius.  Needs cr_sv_build. */
+CR_API cr_status cr_ppm_estimate(cr_context* ctx, const cr_push_constants* frames, uint32_t n_frames,
+                                 const cr_ppm_params* p);
+/* The photon grid: arrivals per cell (as the reference's collisions[cell].x;
+ * n_cells = voxelCount product) and every deposited photon sorted by cell in
+ * emission order (n_deposits); a cell's kept photons are the first
+ * min(arrivals, CR_PPM_PHOTONS_PER_CELL) of its run.  NULL pointers return only
+ * the sizes. */
+CR_API cr_status cr_ppm_download(cr_context* ctx, uint32_t* n_cells, uint32_t* n_deposits, uint32_t* arrivals,
+                                 cr_photon* photons);
+
 /* --------------------------------------------- learned in-scattering (extension)
  * A 224 -> 128 -> 128 -> 128 -> 128 -> 3 ReLU network on bf16 MFMA with fp32
  * accumulation, predicting log(1 + Y) per RGB channel, where Y is the radiance
@@ -440,7 +496,10 @@ CR_API cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t
  * these exact functions; the probe lets a test compare them bit for bit with a
  * CPU implementation.  No reference counterpart: the reference calls GLSL's
  * built-ins, whose precision the Vulkan driver defines. */
-enum { CR_MATH_SQRT = 0, CR_MATH_LOG = 1, CR_MATH_EXP = 2, CR_MATH_SIN = 3, CR_MATH_COS = 4 };
+/* CR_MATH_ATAN2 pairs the inputs: out[i] = atan2(in[i], in[i ^ 1]) (in[i] itself
+ * for the last of an odd count). */
+enum { CR_MATH_SQRT = 0, CR_MATH_LOG = 1, CR_MATH_EXP = 2, CR_MATH_SIN = 3, CR_MATH_COS = 4, CR_MATH_ACOS = 5,
+       CR_MATH_ATAN2 = 6 };
 CR_API cr_status cr_math_probe(cr_context* ctx, int op, const float* in, uint32_t n, float* out);
 
 /* Density mip pyramid (8 levels, 2x2x2 means) of the current grid, used by the

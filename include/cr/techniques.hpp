@@ -195,4 +195,43 @@ private:
     cr_gather_ext gather_{};
 };
 
+// RenderTechniquePPM (RenderTechniquePPM.h:8-67): per frame, UpdateRadius,
+// photon tracing into the photon grid with the frame's seed, then the
+// ray-marched estimate blended into the accumulation image
+// (RenderTechniquePPM.cpp:229-300).
+class RenderTechniquePPM : public RenderTechnique {
+public:
+    RenderTechniquePPM(Device* d, const cr_camera_properties* cam, const cr_photon_map_properties* pm,
+                       cr_push_constants* pc, float initialRadius)
+        : RenderTechnique(d, pc), initialRadius_(initialRadius)
+    {
+        if (cam) QueueUpdateCameraProperties(*cam);
+        if (pm) UpdatePhotonMapProperties(*pm);
+        pc->pmRadius = initialRadius;
+        cr_ppm_params_default(&params_);
+    }
+    void UpdatePhotonMapProperties(const cr_photon_map_properties& pm)
+    {
+        check(dev_->get(), cr_set_photon_map(dev_->get(), &pm), "cr_set_photon_map");
+    }
+    void SetParams(const cr_ppm_params& p) { params_ = p; }
+    // RenderTechniquePPM.cpp:302-312
+    void UpdateRadius(unsigned int frameNumber)
+    {
+        push_->pmRadius = cr_ppm_update_radius(frameNumber, CR_PPM_ALPHA, initialRadius_, push_->pmRadius);
+    }
+    void RecordDrawCommands(unsigned int = 0) override
+    {
+        UpdateDescriptorSets();
+        cr_context* c = dev_->get();
+        UpdateRadius(push_->frameCount);
+        check(c, cr_ppm_trace(c, push_->seed, &params_, nullptr), "cr_ppm_trace");
+        check(c, cr_ppm_estimate(c, push_, 1, &params_), "cr_ppm_estimate");
+    }
+
+private:
+    float initialRadius_;
+    cr_ppm_params params_{};
+};
+
 } // namespace cr

@@ -108,6 +108,30 @@ typedef struct cr_photon_beam_data {
     float trDistances[16];
 } cr_photon_beam_data;
 
+/* UniformBuffers.h:202-210 — one photon of the photon map (48 B; the
+ * reference's comment says 36).  theta = acos(dir.z), phi = atan2(dir.y, dir.x)
+ * in [0, 2 pi) of the photon's propagation direction. */
+typedef struct cr_photon {
+    float position[4];
+    float power[4];
+    float phi;
+    float theta;
+    float _padding[2];
+} cr_photon;
+
+/* UniformBuffers.h:212-249 — photon-map grid (76 B): a cube from the cloud's
+ * lower corner with the cloud's largest extent as side (SetBounds), split in
+ * voxelCount cells of voxelSize; stepSize is the estimate's ray-march step. */
+typedef struct cr_photon_map_properties {
+    float lightDirection[4];
+    float bounds[2][4];
+    int32_t voxelCount[3];
+    float voxelSize;
+    uint32_t photonSize;
+    float stepSize;
+    float absorption;
+} cr_photon_map_properties;
+
 /* UniformBuffers.h:196-200 — radix-sort element (8 B). */
 typedef struct cr_sort_element {
     uint32_t idx;
@@ -144,6 +168,11 @@ CR_STATIC_ASSERT(offsetof(cr_tree_node, bounds) == 16, "bounds@16");
 CR_STATIC_ASSERT(offsetof(cr_tree_node, processed) == 48, "processed@48");
 CR_STATIC_ASSERT(sizeof(cr_photon_beam_data) == 80, "PhotonBeamData is 80 B");
 CR_STATIC_ASSERT(sizeof(cr_sort_element) == 8, "SortElement is 8 B");
+CR_STATIC_ASSERT(sizeof(cr_photon) == 48, "Photon is 48 B");
+CR_STATIC_ASSERT(offsetof(cr_photon, phi) == 32, "phi@32");
+CR_STATIC_ASSERT(sizeof(cr_photon_map_properties) == 76, "PhotonMapProperties is 76 B");
+CR_STATIC_ASSERT(offsetof(cr_photon_map_properties, voxelCount) == 48, "voxelCount@48");
+CR_STATIC_ASSERT(offsetof(cr_photon_map_properties, stepSize) == 68, "stepSize@68");
 CR_STATIC_ASSERT(sizeof(cr_lbvh_push_constants) == 12, "LBVHPushConstants is 12 B");
 
 #ifdef __cplusplus

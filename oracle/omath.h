@@ -25,6 +25,9 @@
  *             loses accuracy there, and the quadrant would need a float to
  *             int conversion that C leaves undefined for huge values).
  *  o_pow15  : x^1.5 = x * sqrt(x) (sqrt is correctly rounded in IEEE-754).
+ *  o_atan2f : octant reduction to atan(t), t in [0, 1]; above tan(pi/12)
+ *             atan(t) = pi/6 + atan((t sqrt3 - 1) / (t + sqrt3)); Taylor series
+ *             to t^13 on |t| <= tan(pi/12).  o_acosf(x) = atan2(sqrt((1-x)(1+x)), x).
  *
  * Compile with -ffp-contract=off and without -ffast-math.
  */
@@ -119,6 +122,38 @@ static inline void o_sincosf(float x, float* s_out, float* c_out)
     else if (q == 2) { *s_out = -sn; *c_out = -cs; }
     else { *s_out = -cs; *c_out = sn; }
 }
+
+/* atan(t) for |t| <= tan(pi/12): Taylor series to t^13 (truncation < 3e-10). */
+static inline float o_atan_small(float t)
+{
+    const float z = t * t;
+    float p = 0.0769230769f;      /*  1/13 */
+    p = p * z - 0.0909090909f;    /* -1/11 */
+    p = p * z + 0.111111111f;     /*  1/9  */
+    p = p * z - 0.142857143f;     /* -1/7  */
+    p = p * z + 0.2f;             /*  1/5  */
+    p = p * z - 0.333333333f;     /* -1/3  */
+    return t + t * (z * p);
+}
+
+/* atan2 by octant: atan of min/max in [0, 1], above tan(pi/12) reduced by
+ * atan(t) = pi/6 + atan((t sqrt3 - 1) / (t + sqrt3)); then pi/2 - a, pi - a and
+ * the sign of y.  atan2(0, 0) = 0.  acos and atan2 <= 3 ULP (tests/test_oracle.py). */
+static inline float o_atan2f(float y, float x)
+{
+    const float ay = fabsf(y), ax = fabsf(x);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mx == 0.0f ? 0.0f : mn / mx;
+    const int big = t > 0.267949194f;
+    const float rt = big ? (t * 1.73205081f - 1.0f) / (t + 1.73205081f) : t;
+    const float at = o_atan_small(rt);
+    float r = big ? 0.523598776f + at : at;
+    r = ay > ax ? 1.57079633f - r : r;
+    r = x < 0.0f ? 3.14159265f - r : r;
+    return y < 0.0f ? -r : r;
+}
+
+static inline float o_acosf(float x) { return o_atan2f(sqrtf((1.0f - x) * (1.0f + x)), x); }
 
 static inline float o_pow15(float x) { return x * sqrtf(x); }
 

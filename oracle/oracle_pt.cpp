@@ -34,6 +34,8 @@ void or_math_array(int op, const float* in, uint32_t n, float* out)
         case CR_MATH_SQRT: out[i] = sqrtf(x); break;
         case CR_MATH_LOG: out[i] = o_logf(x); break;
         case CR_MATH_EXP: out[i] = o_expf(x); break;
+        case CR_MATH_ACOS: out[i] = o_acosf(x); break;
+        case CR_MATH_ATAN2: out[i] = o_atan2f(x, in[(i ^ 1u) < n ? (i ^ 1u) : i]); break;
         default: o_sincosf(x, &sn, &cs); out[i] = op == CR_MATH_SIN ? sn : cs; break;
         }
     }

@@ -44,7 +44,10 @@ class Oracle:
             _fields_ = [("base", OrScene), ("beams", vp), ("data", vp), ("nodes", vp), ("n", C.c_uint32),
                         ("refCompat", C.c_uint32), ("jitter", C.c_uint32), ("albedo", C.c_float)]
 
-        self.OrScene, self.OrCounters, self.OrBeamScene = OrScene, OrCounters, OrBeamScene
+        class OrPhotonMap(C.Structure):
+            _fields_ = [("arrivals", vp), ("start", vp), ("photons", vp)]
+
+        self.OrScene, self.OrCounters, self.OrBeamScene, self.OrPhotonMap = OrScene, OrCounters, OrBeamScene, OrPhotonMap
         sig = {
             "or_logf": (C.c_float, [C.c_float]),
             "or_expf": (C.c_float, [C.c_float]),
@@ -92,6 +95,14 @@ class Oracle:
             "or_gather_render": (None, [P(OrBeamScene), P(S.PushConstants), C.c_uint32, vp, C.c_int32]),
             "or_ppb_update_radius": (C.c_float, [C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_float]),
             "or_sqrt_correction_errors": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+            "or_ppm_trace_count": (None, [P(OrScene), P(S.PhotonMapProperties), P(S.PpmParams), C.c_int32, vp]),
+            "or_ppm_trace_write": (None, [P(OrScene), P(S.PhotonMapProperties), P(S.PpmParams), C.c_int32, vp, vp,
+                                          vp]),
+            "or_ppm_build": (None, [vp, vp, C.c_uint32, C.c_uint32, vp, vp, vp]),
+            "or_ppm_query": (None, [P(OrScene), P(S.PhotonMapProperties), P(OrPhotonMap), P(C.c_float),
+                                    P(C.c_float), C.c_float, C.c_float, P(C.c_float)]),
+            "or_ppm_estimate_render": (None, [P(OrScene), P(S.PhotonMapProperties), P(OrPhotonMap), P(S.PpmParams),
+                                              P(S.PushConstants), C.c_uint32, vp, C.c_int32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -207,6 +218,52 @@ class Oracle:
             r = np.float32(self.lib.or_ppb_update_radius(f, beams_per_pass, alpha, initial, float(r)))
             out[f] = r
         return out
+
+    # ------------------------------------------------------ photon mapping
+    def ppm_trace(self, s, pm, params, seed):
+        """Deposits in emission order: (photons [n] of cr_photon as a structured
+        uint8 view, cells [n])."""
+        P_ = params.numPaths
+        counts = np.empty(P_, np.uint32)
+        self.lib.or_ppm_trace_count(C.byref(s), C.byref(pm), C.byref(params), seed, counts.ctypes.data)
+        offsets = np.zeros(P_, np.uint64)
+        offsets[1:] = np.cumsum(counts[:-1], dtype=np.uint64)
+        n = int(counts.sum())
+        photons = (self.S.Photon * max(n, 1))()
+        cells = np.empty(max(n, 1), np.uint32)
+        self.lib.or_ppm_trace_write(C.byref(s), C.byref(pm), C.byref(params), seed, offsets.ctypes.data, photons,
+                                    cells.ctypes.data)
+        return photons, cells[:n], n
+
+    def ppm_build(self, photons, cells, n, n_cells):
+        arrivals = np.empty(n_cells, np.uint32)
+        start = np.empty(n_cells, np.uint32)
+        sorted_ = (self.S.Photon * max(n, 1))()
+        cells = np.ascontiguousarray(cells, np.uint32)
+        self.lib.or_ppm_build(photons, cells.ctypes.data, n, n_cells, arrivals.ctypes.data, start.ctypes.data, sorted_)
+        return arrivals, start, sorted_
+
+    def photon_map(self, arrivals, start, sorted_):
+        m = self.OrPhotonMap()
+        m._keep = [arrivals, start, sorted_]
+        m.arrivals = arrivals.ctypes.data
+        m.start = start.ctypes.data
+        m.photons = C.addressof(sorted_)
+        return m
+
+    def ppm_query(self, s, pm, m, pos, dir_, scatter, radius):
+        out = (C.c_float * 4)()
+        f3 = C.c_float * 3
+        self.lib.or_ppm_query(C.byref(s), C.byref(pm), C.byref(m), f3(*pos), f3(*dir_), scatter, radius, out)
+        return np.array(out, np.float32)
+
+    def ppm_estimate(self, s, pm, m, params, frames, accum=None, threads=0):
+        H, W = 2 * s.cam.halfHeight, 2 * s.cam.halfWidth
+        a = np.zeros((H, W, 4), np.float32) if accum is None else np.ascontiguousarray(accum, np.float32).copy()
+        arr = (self.S.PushConstants * len(frames))(*frames)
+        self.lib.or_ppm_estimate_render(C.byref(s), C.byref(pm), C.byref(m), C.byref(params), arr, len(frames),
+                                        a.ctypes.data, threads)
+        return a
 
     # ---------------------------------------------------------------- LBVH
     def lbvh_build(self, beams, b0, b1):

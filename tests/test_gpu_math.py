@@ -60,3 +60,18 @@ def test_probe_rejects_unknown_op(pkg, ctx):
     x = np.ones(4, np.float32)
     out = np.empty_like(x)
     assert pkg.load().cr_math_probe(ctx.h, 9, x.ctypes.data, 4, out.ctypes.data) != 0
+
+
+@pytest.mark.parametrize("op", [5, 6])
+def test_inverse_trig_bitexact(pkg, ctx, O, op):
+    """acos and atan2 (the photon map's direction angles) on the device
+    against the oracle, bit for bit; atan2 pairs neighbouring inputs."""
+    rng = np.random.default_rng(30 + op)
+    if op == 5:
+        x = np.concatenate([rng.uniform(-1.0, 1.0, 1 << 20), [-1.0, 1.0, 0.0, -0.0, 0.5, -0.5, 0.99999994, -0.99999994]])
+    else:
+        mag = 10.0 ** rng.uniform(-6, 6, 1 << 20)
+        x = np.concatenate([rng.standard_normal(1 << 20) * mag, [0.0, 0.0, 1.0, 0.0, 0.0, -1.0, -1.0, 0.0, 1.0, 1.0,
+                                                                  -1.0, -1.0, 3.0, -3.0]])
+    x = x.astype(np.float32)
+    assert_bitexact(probe(pkg, ctx, op, x), O.math_array(op, x), f"device vs oracle, op {op}")

@@ -102,6 +102,32 @@ def test_sincosf_accuracy(O):
     assert es < 2.5e-7 and ec < 2.5e-7
 
 
+def ulp_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    return np.abs(got - ref) / ulp
+
+
+def test_acos_atan2_accuracy(O):
+    """o_acosf and o_atan2f (omath.h) within 3 ULP of float64 arccos / arctan2 (acos
+    reaches 2.5 ULP just below 1, where sqrt((1-x)(1+x)) is small),
+    every octant and the axes."""
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.uniform(-1.0, 1.0, 200000), [-1.0, 1.0, 0.0, 0.5, -0.5, 0.7071068]]).astype(np.float32)
+    got = O.math_array(5, x)
+    assert ulp_err(got, np.arccos(x.astype(np.float64))).max() <= 3.0
+    y = (rng.standard_normal(200000) * 10.0 ** rng.uniform(-4, 4, 200000)).astype(np.float32)
+    xx = (rng.standard_normal(200000) * 10.0 ** rng.uniform(-4, 4, 200000)).astype(np.float32)
+    pairs = np.empty(400000, np.float32)
+    pairs[0::2], pairs[1::2] = y, xx
+    got = O.math_array(6, pairs)[0::2]
+    ref = np.arctan2(y.astype(np.float64), xx.astype(np.float64))
+    big = np.abs(ref) > 1e-30
+    assert ulp_err(got[big], ref[big]).max() <= 3.0
+    axes = np.array([0.0, 1.0, 1.0, 0.0, 0.0, -1.0, -1.0, 0.0, 0.0, 0.0], np.float32)
+    np.testing.assert_allclose(O.math_array(6, axes)[0::2], [0.0, np.pi / 2, np.pi, -np.pi / 2, 0.0], rtol=1e-7)
+
+
 def test_math_array_domain(O):
     """sin/cos are defined on |x| <= 2^17 and NaN beyond; the array entry
     point is the scalar functions element by element, sqrt is IEEE's."""
