@@ -11,7 +11,8 @@ import os
 from . import structs as S
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcrmi355x.so")
+# CR_LIB_PATH: another build of the same sources (the host-sanitizer build)
+LIB_PATH = os.environ.get("CR_LIB_PATH") or os.path.join(HERE, "libcrmi355x.so")
 
 P = C.POINTER
 vp = C.c_void_p

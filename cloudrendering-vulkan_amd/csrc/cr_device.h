@@ -249,6 +249,15 @@ __device__ __forceinline__ uint64_t voxel_index(const GridView& g, int32_t x, in
     return (brick << 6) + (uint64_t)((x & 3) + ((y & 3) << 2) + ((z & 3) << 4));
 }
 
+// Two binary32 values -> two bf16 packed low/high (v_cvt_pk_bf16_f32, round
+// to nearest even; the learned network's activations and gradients).
+typedef __bf16 crd_v2bf __attribute__((ext_vector_type(2)));
+typedef float crd_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16x2_rne(float lo, float hi)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((crd_v2f{lo, hi}), crd_v2bf));
+}
+
 // binary32 -> binary16 bits, round to nearest, ties to even, in integer
 // arithmetic.  Handing a float expression straight to a half conversion lets
 // the compiler fold its last multiply into v_fma_mixlo_f16, which rounds the

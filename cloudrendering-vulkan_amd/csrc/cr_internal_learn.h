@@ -62,7 +62,5 @@ hipError_t launch_select(const float* src, uint32_t stride, uint32_t count, floa
 // the targets leave out nee's term of each record.
 hipError_t launch_append(const uint16_t* X, const float4* T4, const float* pts, const FirstNee& nee, uint32_t stride,
                          uint32_t count, uint16_t* dsX, float* dsT, uint64_t offset, hipStream_t s);
-hipError_t launch_batch_gather(const uint16_t* dsX, const float* dsT, uint64_t n, uint32_t seed, uint32_t step, uint32_t B,
-                               uint16_t* X, float* T, hipStream_t s);
 
 } // namespace cr

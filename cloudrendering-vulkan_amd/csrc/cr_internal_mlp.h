@@ -14,33 +14,55 @@ constexpr int MLP_LAYERS = 5;
 constexpr uint32_t MLP_K[MLP_LAYERS] = {224, 128, 128, 128, 128};
 constexpr uint32_t MLP_N[MLP_LAYERS] = {128, 128, 128, 128, 16};
 
+// Rows per training tile (one workgroup of the fused training kernel).
+constexpr uint32_t MLP_TILE = 32;
+inline uint64_t mlp_padded_rows(uint64_t B) { return (B + MLP_TILE - 1) / MLP_TILE * MLP_TILE; }
+inline unsigned mlp_tiles(uint64_t B) { return (unsigned)((B + MLP_TILE - 1) / MLP_TILE); }
+
+// The training kernel keeps, for the weight-gradient GEMM, every layer's input
+// and every layer's output gradient feature-major ([feature][Bp] bf16, Bp the
+// padded row count): X^T (224 rows), H1..H4^T (128 each), dZ0..dZ3^T (128
+// each), dZ4^T (16).  Offsets in elements, in units of Bp.
+constexpr uint32_t MLP_TR_IN[MLP_LAYERS] = {0, 224, 352, 480, 608};
+constexpr uint32_t MLP_TR_DZ[MLP_LAYERS] = {736, 864, 992, 1120, 1248};
+constexpr uint32_t MLP_TR_ROWS = 1264;
+
+// Gradient slots of the weight-gradient GEMM: the batch's 32-row chunks are
+// split into `slots` contiguous ranges, each summed into a slot of its own and
+// the slots added in order by the Adam kernel (no float atomics).
+constexpr uint32_t MLP_MAX_SLOTS = 16;
+inline uint32_t mlp_slots(uint64_t B)
+{
+    const uint64_t s = (mlp_tiles(B) + 31) / 32;
+    return (uint32_t)(s < 1 ? 1 : (s > MLP_MAX_SLOTS ? MLP_MAX_SLOTS : s));
+}
+// Output tiles of the weight-gradient GEMM (32 x 32, 32 x 16 for the last layer).
+constexpr uint32_t MLP_GRAD_TILES = 7 * 4 + 3 * 4 * 4 + 4;
+
 struct MlpArgs {
-    const uint16_t* X;       // [B][224] bf16 features
+    // batch source: X [B][224] bf16 and T [B][3], or, when dsX is set, rows
+    // drawn from the data set (dsX [dsN][224], dsT [dsN][3]) by (seed, step)
+    const uint16_t* X;
+    const float* T;
+    const uint16_t* dsX;
+    const float* dsT;
+    uint64_t dsN;
+    uint32_t seed, step;
     int64_t B;
-    const uint32_t* Bdev;    // forward only: when set, the row count is *Bdev (<= B, B sizes the launch)
-    uint16_t* WT[MLP_LAYERS];  // bf16 [N][K] (forward B fragments)
-    uint16_t* W[MLP_LAYERS];   // bf16 [K][N] (backward B fragments)
+    uint64_t Bp;             // padded row count (a multiple of 32)
+    const uint32_t* Bdev;    // inference only: when set, the row count is *Bdev (<= B, B sizes the launch)
+    uint16_t* WT[MLP_LAYERS];  // bf16 [N][K]
+    uint16_t* W[MLP_LAYERS];   // bf16 [K][N]
     const float* bias[MLP_LAYERS];
-    float* Y;                // [B][4] fp32 outputs (log(1 + radiance), 3 used)
-    uint16_t* acts;          // [4][B][128] bf16 hidden activations (training)
-    size_t actStride;        // elements between layers in acts
-    const float* dY;         // [B][4] fp32
+    float* Y;                // inference: [B][4] fp32 outputs (log(1 + radiance), 3 used)
+    uint16_t* tr;            // training: feature-major inputs and output gradients (MLP_TR_ROWS x Bp)
+    float* loss;             // training: per-tile loss shares, added in order by the host
     float* grad;             // flat fp32 gradients (same layout as the parameters), written by the Adam kernel
-    float* partial;          // [nWaves][total] per-wave gradient sums of the backward pass
-    uint32_t nWaves;         // backward waves: wave w takes the 32-row chunks w, w + nWaves, ...
+    float* slot;             // [slots][total] gradient sums of the weight-gradient GEMM
+    uint32_t slots;
     uint32_t total;          // flat parameter count
     uint32_t wOff[MLP_LAYERS], bOff[MLP_LAYERS];
 };
-
-// Backward waves for a batch of B rows: one per 32-row chunk, at most MLP_MAX_WAVES.
-constexpr uint32_t MLP_MAX_WAVES = 1024;
-inline uint32_t mlp_waves(uint64_t B)
-{
-    const uint64_t c = (B + 31) / 32;
-    return (uint32_t)(c < MLP_MAX_WAVES ? c : MLP_MAX_WAVES);
-}
-// Loss kernel blocks (one partial loss each) for a batch of B rows.
-inline unsigned mlp_loss_blocks(int64_t B) { return (unsigned)((B + 255) / 256); }
 
 struct AdamArgs {
     float* params;
@@ -57,10 +79,23 @@ struct AdamArgs {
 };
 
 hipError_t launch_mlp_probe(const uint16_t* A, const uint16_t* B, float* D, hipStream_t s);
-hipError_t launch_mlp_forward(const MlpArgs& a, bool train, hipStream_t s);
-// loss: mlp_loss_blocks(B) partial sums, to be added in order
-hipError_t launch_mlp_loss(const float* Y, const float* T, int64_t B, float* dY, float* loss, hipStream_t s);
-hipError_t launch_mlp_backward(const MlpArgs& a, hipStream_t s);
+// Inference: persistent, `blocks` workgroups (one per CU).
+hipError_t launch_mlp_infer(const MlpArgs& a, unsigned blocks, hipStream_t s);
+// Training: the fused forward / loss / backward chain (mlp_tiles(B) loss
+// shares), then the weight-gradient GEMM into a.slots slots, then Adam.
+hipError_t launch_mlp_train(const MlpArgs& a, hipStream_t s);
+hipError_t launch_mlp_wgrad(const MlpArgs& a, hipStream_t s);
 hipError_t launch_mlp_adam(const MlpArgs& a, const AdamArgs& o, hipStream_t s);
+
+// The data-set row a training step draws for batch row r (shared by the
+// training kernel and the host-side description in cr_learn_train).
+__host__ __device__ inline uint64_t mlp_draw(uint32_t seed, uint32_t step, uint32_t r, uint64_t n)
+{
+    uint32_t state = (seed * 0x9E3779B9u) ^ (step * 0x85EBCA6Bu) ^ (r * 0xC2B2AE35u);
+    state = state * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    word = (word >> 22u) ^ word;
+    return (uint64_t)word % n;
+}
 
 } // namespace cr

@@ -393,7 +393,7 @@ cr_status cr_grid_skip(cr_context* ctx, float* out)
 cr_status cr_math_probe(cr_context* ctx, int op, const float* in, uint32_t n, float* out)
 {
     CR_CHECK_CTX(ctx);
-    if (op < CR_MATH_SQRT || op > CR_MATH_ATAN2) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "unknown math op");
+    if (op < CR_MATH_SQRT || op > CR_MATH_BF16) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "unknown math op");
     if (n == 0) return CR_OK;
     if (!in || !out) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null array");
     float* d = nullptr;

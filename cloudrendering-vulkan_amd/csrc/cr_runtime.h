@@ -122,7 +122,16 @@ bool mlp_initialized(cr_context* ctx);
 void learn_network_reset(cr_context* ctx);
 void learn_free(cr_context* ctx);
 void batch_free(cr_context* ctx);
-cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr);
+// Rows drawn from a device-resident data set for one training step: batch row r
+// takes data-set row mlp_draw(seed, step, r, n) (cr_internal_mlp.h).
+struct MlpDraw {
+    const uint16_t* X;  // [n][224] bf16 features
+    const float* T;     // [n][3] targets
+    uint64_t n;
+    uint32_t seed, step;
+};
+cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr,
+                           const MlpDraw* draw = nullptr);
 // Bdev: optional device-resident row count (<= B; B sizes the launch).
 cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, float* Y, const uint32_t* Bdev = nullptr);
 cr_status mlp_last_loss(cr_context* ctx, float* loss);

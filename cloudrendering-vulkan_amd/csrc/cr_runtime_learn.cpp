@@ -35,14 +35,10 @@ struct Learn {
     size_t YCap = 0;
     float* targets = nullptr;
     size_t targetsCap = 0;
-    // training data set and the minibatch buffers
+    // training data set; step s of cr_learn_train draws its rows with step number trainSteps + s
     uint16_t* dsX = nullptr;
     float* dsT = nullptr;
     uint64_t dsN = 0, dsCap = 0;
-    uint16_t* bX = nullptr;
-    size_t bXCap = 0;
-    float* bT = nullptr;
-    size_t bTCap = 0;
     uint32_t trainSteps = 0;
     // whether the data set's targets leave out the first collision's
     // shadow-volume term (-1: empty data set), and the same for the data the
@@ -82,8 +78,7 @@ void learn_free(cr_context* ctx)
     if (!L) return;
     for (int k = 0; k < LEARN_LEVELS; k++)
         if (L->mips[k]) hipFree(L->mips[k]);
-    void* bufs[] = {L->points, L->X, L->records, L->slot, L->counts, L->mdev, L->Y, L->targets, L->dsX, L->dsT, L->bX,
-                    L->bT};
+    void* bufs[] = {L->points, L->X, L->records, L->slot, L->counts, L->mdev, L->Y, L->targets, L->dsX, L->dsT};
     for (void* b : bufs)
         if (b) hipFree(b);
     delete L;
@@ -437,12 +432,9 @@ cr_status cr_learn_train(cr_context* ctx, uint32_t steps, uint32_t batch, float 
     Learn* L = learn_get(ctx);
     if (L->dsN == 0) return fail(ctx, CR_ERR_STATE, "empty training data set (cr_learn_collect)");
     if (!mlp_initialized(ctx)) CR_TRY(cr_mlp_init(ctx, seed));
-    CR_TRY(ensure_buffer(ctx, (void**)&L->bX, &L->bXCap, (size_t)batch * 448, "batch X"));
-    CR_TRY(ensure_buffer(ctx, (void**)&L->bT, &L->bTCap, (size_t)batch * 12, "batch T"));
     for (uint32_t s = 0; s < steps; s++) {
-        CR_HIP(ctx, launch_batch_gather(L->dsX, L->dsT, L->dsN, seed, L->trainSteps + s, batch, L->bX, L->bT, ctx->stream),
-               "batch gather");
-        CR_TRY(mlp_train_device(ctx, L->bX, L->bT, batch, lr));
+        const MlpDraw draw = {L->dsX, L->dsT, L->dsN, seed, L->trainSteps + s};
+        CR_TRY(mlp_train_device(ctx, nullptr, nullptr, batch, lr, &draw));
     }
     L->trainSteps += steps;
     L->netFirstNee = L->dsFirstNee;

@@ -30,15 +30,13 @@ struct Mlp {
     size_t TCap = 0;
     float* Y = nullptr;
     size_t YCap = 0;
-    float* dY = nullptr;
-    size_t dYCap = 0;
-    uint16_t* acts = nullptr;
-    size_t actsCap = 0;
-    float* loss = nullptr;     // the last training step's per-block loss sums
+    uint16_t* tr = nullptr;    // feature-major layer inputs and output gradients of the last step
+    size_t trCap = 0;
+    float* loss = nullptr;     // the last training step's per-tile loss shares
     size_t lossCap = 0;
     unsigned lossBlocks = 0;
-    float* partial = nullptr;  // per-wave gradient sums of the backward pass
-    size_t partialCap = 0;
+    float* slot = nullptr;     // gradient slots of the weight-gradient GEMM
+    size_t slotCap = 0;
     // moving average of the parameters (allocated when first turned on)
     float emaDecay = 0.0f;     // 0: off
     uint64_t emaSteps = 0;     // steps averaged since the last restart
@@ -126,17 +124,17 @@ cr_status upload_params(cr_context* ctx, Mlp* m, const float* p)
     return CR_OK;
 }
 
-cr_status ensure_batch(cr_context* ctx, Mlp* m, uint64_t B, bool train)
+cr_status ensure_batch(cr_context* ctx, Mlp* m, uint64_t B, bool train, bool hostX)
 {
-    cr_status st = ensure_buffer(ctx, (void**)&m->X, &m->XCap, B * 224 * 2, "mlp X");
-    if (st == CR_OK) st = ensure_buffer(ctx, (void**)&m->Y, &m->YCap, B * 16, "mlp Y");
+    cr_status st = CR_OK;
+    if (hostX) st = ensure_buffer(ctx, (void**)&m->X, &m->XCap, B * 224 * 2, "mlp X");
+    if (st == CR_OK && !train) st = ensure_buffer(ctx, (void**)&m->Y, &m->YCap, B * 16, "mlp Y");
+    if (st == CR_OK && train) st = ensure_buffer(ctx, (void**)&m->loss, &m->lossCap, (size_t)mlp_tiles(B) * 4, "mlp loss");
     if (st == CR_OK && train)
-        st = ensure_buffer(ctx, (void**)&m->loss, &m->lossCap, (size_t)mlp_loss_blocks((int64_t)B) * 4, "mlp loss");
+        st = ensure_buffer(ctx, (void**)&m->slot, &m->slotCap, (size_t)mlp_slots(B) * m->total * 4, "mlp gradient slots");
+    if (st == CR_OK && train && hostX) st = ensure_buffer(ctx, (void**)&m->T, &m->TCap, B * 12, "mlp targets");
     if (st == CR_OK && train)
-        st = ensure_buffer(ctx, (void**)&m->partial, &m->partialCap, (size_t)mlp_waves(B) * m->total * 4, "mlp partial gradients");
-    if (st == CR_OK && train) st = ensure_buffer(ctx, (void**)&m->T, &m->TCap, B * 12, "mlp targets");
-    if (st == CR_OK && train) st = ensure_buffer(ctx, (void**)&m->dY, &m->dYCap, B * 16, "mlp dY");
-    if (st == CR_OK && train) st = ensure_buffer(ctx, (void**)&m->acts, &m->actsCap, B * 128 * 2 * 4, "mlp acts");
+        st = ensure_buffer(ctx, (void**)&m->tr, &m->trCap, (size_t)MLP_TR_ROWS * mlp_padded_rows(B) * 2, "mlp training activations");
     return st;
 }
 
@@ -155,6 +153,7 @@ MlpArgs args_of(Mlp* m, const uint16_t* X, uint64_t B)
     memset(&a, 0, sizeof(a));
     a.X = X;
     a.B = (int64_t)B;
+    a.Bp = mlp_padded_rows(B);
     for (int l = 0; l < MLP_LAYERS; l++) {
         a.WT[l] = m->WT[l];
         a.W[l] = m->W[l];
@@ -163,12 +162,11 @@ MlpArgs args_of(Mlp* m, const uint16_t* X, uint64_t B)
         a.bOff[l] = m->bOff[l];
     }
     a.Y = m->Y;
-    a.acts = m->acts;
-    a.actStride = (size_t)B * 128;
-    a.dY = m->dY;
+    a.tr = m->tr;
+    a.loss = m->loss;
     a.grad = m->grad;
-    a.partial = m->partial;
-    a.nWaves = mlp_waves(B);
+    a.slot = m->slot;
+    a.slots = mlp_slots(B);
     a.total = m->total;
     return a;
 }
@@ -196,7 +194,7 @@ void mlp_free(cr_context* ctx)
 {
     Mlp* m = ctx->mlp;
     if (!m) return;
-    void* bufs[] = {m->params, m->grad, m->m, m->v, m->X, m->T, m->Y, m->dY, m->acts, m->loss, m->partial, m->ema, m->emaHat};
+    void* bufs[] = {m->params, m->grad, m->m, m->v, m->X, m->T, m->Y, m->tr, m->loss, m->slot, m->ema, m->emaHat};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (int l = 0; l < MLP_LAYERS; l++) {
@@ -218,7 +216,7 @@ cr_status mlp_forward_device(cr_context* ctx, const uint16_t* X, uint64_t B, flo
     MlpArgs a = inference_args(m, X, B);
     a.Y = Y;
     a.Bdev = Bdev;
-    hipError_t e = launch_mlp_forward(a, false, ctx->stream);
+    hipError_t e = launch_mlp_infer(a, (unsigned)ctx->numCUs, ctx->stream);
     return e == hipSuccess ? CR_OK : hip_fail(ctx, e, "mlp forward");
 }
 } // namespace cr
@@ -359,10 +357,10 @@ cr_status cr_mlp_forward(cr_context* ctx, const uint16_t* X, uint32_t B, float* 
     Mlp* m;
     CR_TRY(mlp_get(ctx, &m));
     if (!m->initialized) return fail(ctx, CR_ERR_STATE, "no network parameters (cr_mlp_init / cr_mlp_set_params)");
-    CR_TRY(ensure_batch(ctx, m, B, false));
+    CR_TRY(ensure_batch(ctx, m, B, false, true));
     CR_HIP(ctx, hipMemcpyAsync(m->X, X, (size_t)B * 224 * 2, hipMemcpyHostToDevice, ctx->stream), "copy X");
     MlpArgs a = inference_args(m, m->X, B);
-    CR_HIP(ctx, launch_mlp_forward(a, false, ctx->stream), "mlp forward");
+    CR_HIP(ctx, launch_mlp_infer(a, (unsigned)ctx->numCUs, ctx->stream), "mlp forward");
     std::vector<float> y((size_t)B * 4);
     CR_HIP(ctx, hipMemcpyAsync(y.data(), m->Y, y.size() * 4, hipMemcpyDeviceToHost, ctx->stream), "copy Y");
     CR_HIP(ctx, hipStreamSynchronize(ctx->stream), "sync");
@@ -381,7 +379,7 @@ cr_status cr_mlp_train_step(cr_context* ctx, const uint16_t* X, const float* tar
     Mlp* m;
     CR_TRY(mlp_get(ctx, &m));
     if (!m->initialized) return fail(ctx, CR_ERR_STATE, "no network parameters (cr_mlp_init / cr_mlp_set_params)");
-    CR_TRY(ensure_batch(ctx, m, B, true));
+    CR_TRY(ensure_batch(ctx, m, B, true, true));
     hipStream_t s = ctx->stream;
     CR_HIP(ctx, hipMemcpyAsync(m->X, X, (size_t)B * 224 * 2, hipMemcpyHostToDevice, s), "copy X");
     CR_HIP(ctx, hipMemcpyAsync(m->T, targets, (size_t)B * 12, hipMemcpyHostToDevice, s), "copy targets");
@@ -400,12 +398,12 @@ cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t* B, fl
     if (!A || !B || !D) return fail(ctx, CR_ERR_INVALID_ARGUMENT, "null matrix");
     hipSetDevice(ctx->device);
     void* buf = nullptr;
-    CR_HIP(ctx, hipMalloc(&buf, 2048), "hipMalloc");
+    CR_HIP(ctx, hipMalloc(&buf, 3072), "hipMalloc");
     uint16_t* dA = (uint16_t*)buf;
-    uint16_t* dB = dA + 256;
-    float* dD = (float*)(dB + 256);
-    hipError_t e = hipMemcpyAsync(dA, A, 512, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(dB, B, 512, hipMemcpyHostToDevice, ctx->stream);
+    uint16_t* dB = dA + 512;
+    float* dD = (float*)(dB + 512);
+    hipError_t e = hipMemcpyAsync(dA, A, 1024, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dB, B, 1024, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = launch_mlp_probe(dA, dB, dD, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(D, dD, 1024, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -433,18 +431,29 @@ cr_status mlp_last_loss(cr_context* ctx, float* loss)
     return CR_OK;
 }
 
-// Forward (training), loss, backward and Adam on device-resident X and T.
-cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr)
+// One training step on device-resident X and T, or on rows drawn from a data
+// set (draw): the fused forward / loss / backward kernel, the weight-gradient
+// GEMM, then Adam.
+cr_status mlp_train_device(cr_context* ctx, const uint16_t* X, const float* T, uint64_t B, float lr, const MlpDraw* draw)
 {
     Mlp* m;
     CR_TRY(mlp_get(ctx, &m));
-    CR_TRY(ensure_batch(ctx, m, B, true));
+    CR_TRY(ensure_batch(ctx, m, B, true, false));
     hipStream_t s = ctx->stream;
     MlpArgs a = args_of(m, X, B);
-    CR_HIP(ctx, launch_mlp_forward(a, true, s), "mlp forward");
-    CR_HIP(ctx, launch_mlp_loss(m->Y, T, (int64_t)B, m->dY, m->loss, s), "mlp loss");
-    m->lossBlocks = mlp_loss_blocks((int64_t)B);
-    CR_HIP(ctx, launch_mlp_backward(a, s), "mlp backward");
+    a.T = T;
+    if (draw) {
+        a.X = nullptr;
+        a.T = nullptr;
+        a.dsX = draw->X;
+        a.dsT = draw->T;
+        a.dsN = draw->n;
+        a.seed = draw->seed;
+        a.step = draw->step;
+    }
+    CR_HIP(ctx, launch_mlp_train(a, s), "mlp train");
+    m->lossBlocks = mlp_tiles(B);
+    CR_HIP(ctx, launch_mlp_wgrad(a, s), "mlp weight gradients");
     m->step++;
     AdamArgs o;
     o.params = m->params;

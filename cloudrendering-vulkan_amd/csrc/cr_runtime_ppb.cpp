@@ -254,7 +254,6 @@ cr_status cr_lbvh_build(cr_context* ctx)
     if (!p || p->n == 0) return fail(ctx, CR_ERR_STATE, "no beams");
     hipSetDevice(ctx->device);
     const uint32_t n = p->n;
-    const size_t nodes = 2 * (size_t)n - 1;
     CR_TRY(lbvh_buffers(ctx, p, n));
     BoxArg box;
     for (int k = 0; k < 3; k++) {

@@ -306,6 +306,7 @@ __global__ void __launch_bounds__(256) math_probe_kernel(int op, const float* in
     else if (op == CR_MATH_EXP) r = crd_expf(x);
     else if (op == CR_MATH_ACOS) r = crd_acosf(x);
     else if (op == CR_MATH_ATAN2) r = crd_atan2f(x, in[(i ^ 1u) < n ? (i ^ 1u) : i]);
+    else if (op == CR_MATH_BF16) r = __uint_as_float(bf16x2_rne(x, 0.0f) << 16);
     else { crd_sincosf(x, &sn, &cs); r = op == CR_MATH_SIN ? sn : cs; }
     out[i] = r;
 }

@@ -260,22 +260,6 @@ __global__ void __launch_bounds__(256) append_kernel(const uint16_t* X, const fl
 }
 
 // A training batch: row r is data-set row hash(seed, step, r) mod n.
-__global__ void __launch_bounds__(256) batch_gather_kernel(const uint16_t* dsX, const float* dsT, uint64_t n, uint32_t seed,
-                                                           uint32_t step, uint32_t B, uint16_t* X, float* T)
-{
-    const uint32_t r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= B) return;
-    uint32_t state = (seed * 0x9E3779B9u) ^ (step * 0x85EBCA6Bu) ^ (r * 0xC2B2AE35u);
-    state = state * 747796405u + 2891336453u;
-    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
-    word = (word >> 22u) ^ word;
-    const uint64_t src = (uint64_t)word % n;
-    const uint4* xs = reinterpret_cast<const uint4*>(dsX + src * 224);
-    uint4* xd = reinterpret_cast<uint4*>(X + (uint64_t)r * 224);
-    for (int q = 0; q < 28; q++) xd[q] = xs[q];
-    for (int c = 0; c < 3; c++) T[(uint64_t)r * 3 + c] = dsT[src * 3 + c];
-}
-
 } // namespace
 
 namespace cr {
@@ -313,13 +297,6 @@ hipError_t launch_append(const uint16_t* X, const float4* T4, const float* pts, 
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(append_kernel, dim3((count + 255) / 256), dim3(256), 0, s, X, T4, pts, nee, stride, count, dsX, dsT,
                        offset);
-    return hipGetLastError();
-}
-
-hipError_t launch_batch_gather(const uint16_t* dsX, const float* dsT, uint64_t n, uint32_t seed, uint32_t step, uint32_t B,
-                               uint16_t* X, float* T, hipStream_t s)
-{
-    hipLaunchKernelGGL(batch_gather_kernel, dim3((B + 255) / 256), dim3(256), 0, s, dsX, dsT, n, seed, step, B, X, T);
     return hipGetLastError();
 }
 

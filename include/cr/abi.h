@@ -488,7 +488,7 @@ CR_API cr_status cr_mlp_train_step(cr_context* ctx, const uint16_t* X, const flo
 CR_API cr_status cr_mlp_set_ema(cr_context* ctx, float decay);
 /* The bias-corrected average (padded layout); CR_ERR_STATE before any averaged step. */
 CR_API cr_status cr_mlp_get_ema(cr_context* ctx, float* params, uint32_t n);
-/* One 16x16x16 bf16 MFMA on row-major A (16x16) and B (16x16): D = A B. */
+/* One 16x16x32 bf16 MFMA on row-major A (16x32) and B (32x16): D = A B (16x16). */
 CR_API cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t* B, float* D);
 
 /* The kernels' elementary functions applied to n floats on the device
@@ -497,9 +497,10 @@ CR_API cr_status cr_mlp_probe(cr_context* ctx, const uint16_t* A, const uint16_t
  * CPU implementation.  No reference counterpart: the reference calls GLSL's
  * built-ins, whose precision the Vulkan driver defines. */
 /* CR_MATH_ATAN2 pairs the inputs: out[i] = atan2(in[i], in[i ^ 1]) (in[i] itself
- * for the last of an odd count). */
+ * for the last of an odd count).  CR_MATH_BF16: in[i] rounded to bf16 by the
+ * learned network's conversion, returned as the float it represents. */
 enum { CR_MATH_SQRT = 0, CR_MATH_LOG = 1, CR_MATH_EXP = 2, CR_MATH_SIN = 3, CR_MATH_COS = 4, CR_MATH_ACOS = 5,
-       CR_MATH_ATAN2 = 6 };
+       CR_MATH_ATAN2 = 6, CR_MATH_BF16 = 7 };
 CR_API cr_status cr_math_probe(cr_context* ctx, int op, const float* in, uint32_t n, float* out);
 
 /* Density mip pyramid (8 levels, 2x2x2 means) of the current grid, used by the

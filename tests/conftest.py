@@ -31,7 +31,8 @@ def cfg(pkg):
 
 @pytest.fixture(scope="session")
 def O(pkg):
-    return cr_import.oracle()
+    # CR_ORACLE_LIB: another build of the oracle's sources (the sanitizer build)
+    return cr_import.oracle(os.environ.get("CR_ORACLE_LIB"))
 
 
 @pytest.fixture(scope="session")

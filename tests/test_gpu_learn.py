@@ -204,3 +204,52 @@ def test_trained_network_beats_plain_tracer_at_equal_spp(ctx, cfg, S):
     print(f"relMSE at 4 spp: plain {e_plain:.4g}, learned {e_learned:.4g} ({n} training samples, loss {last:.4g})")
     assert e_learned < e_plain
     ctx.learn_dataset_clear()
+
+
+def draw_rows(seed, step, B, n):
+    """The data-set row each batch row of a training step takes: a PCG-style
+    hash of (seed, step, row) modulo the data-set size, in uint32 arithmetic."""
+    r = np.arange(B, dtype=np.uint64)
+    M = np.uint64(0xFFFFFFFF)
+    st = ((np.uint64(seed * 0x9E3779B9 & 0xFFFFFFFF)) ^ np.uint64(step * 0x85EBCA6B & 0xFFFFFFFF)
+          ^ ((r * np.uint64(0xC2B2AE35)) & M))
+    st = (st * np.uint64(747796405) + np.uint64(2891336453)) & M
+    w = (((st >> ((st >> np.uint64(28)) + np.uint64(4))) ^ st) * np.uint64(277803737)) & M
+    w = (w >> np.uint64(22)) ^ w
+    return (w % np.uint64(n)).astype(np.int64)
+
+
+def test_training_draws_the_stated_rows(pkg, cfg, S):
+    """cr_learn_train's step s (counted from the context's first) trains on
+    data-set rows draw_rows(seed, s, B, n), exactly as cr_mlp_train_step does
+    on those rows handed over from the host: the parameters after one drawn
+    step equal, bit for bit, those after one step on the host-gathered batch."""
+    c = pkg.Context(0)
+    try:
+        lib = c.lib
+        sc = scene(cfg, S, w=24, h=20, seed=3)
+        sc.setup(c)
+        n = c.learn_collect(cfg.frames(2), 8, 0)
+        assert n > 50
+        X, T = c.learn_dataset()
+        c.check(lib.cr_mlp_init(c.h, 5), "init")
+        total = C.c_uint32()
+        lib.cr_mlp_param_layout(None, None, None, None, C.byref(total))
+        P0 = np.zeros(total.value, np.float32)
+        c.check(lib.cr_mlp_get_params(c.h, P0.ctypes.data, total.value), "params")
+        B, seed = 300, 17
+        c.check(lib.cr_learn_train(c.h, 1, B, 1e-3, seed, None), "drawn step")
+        P1 = np.zeros_like(P0)
+        c.check(lib.cr_mlp_get_params(c.h, P1.ctypes.data, total.value), "params")
+        idx = draw_rows(seed, 0, B, n)
+        Xb = np.ascontiguousarray(X[idx])
+        Tb = np.ascontiguousarray(T[idx])
+        c.check(lib.cr_mlp_set_params(c.h, P0.ctypes.data, total.value), "reset")
+        loss = C.c_float()
+        c.check(lib.cr_mlp_train_step(c.h, Xb.ctypes.data, Tb.ctypes.data, B, 1e-3, C.byref(loss), None), "host step")
+        P2 = np.zeros_like(P0)
+        c.check(lib.cr_mlp_get_params(c.h, P2.ctypes.data, total.value), "params")
+        assert not np.array_equal(P1, P0)
+        assert np.array_equal(P1.view(np.uint32), P2.view(np.uint32))
+    finally:
+        c.close()

@@ -75,3 +75,30 @@ def test_inverse_trig_bitexact(pkg, ctx, O, op):
                                                                   -1.0, -1.0, 3.0, -3.0]])
     x = x.astype(np.float32)
     assert_bitexact(probe(pkg, ctx, op, x), O.math_array(op, x), f"device vs oracle, op {op}")
+
+
+def test_bf16_conversion_rounds_to_nearest_even(pkg, ctx):
+    """The learned network's float -> bf16 conversion (v_cvt_pk_bf16_f32)
+    rounds to nearest, ties to even, as the oracle's bf16 rounding does: over
+    random bit patterns of normal floats of both signs, exact ties with an even
+    and an odd last kept bit, and values that round up to infinity.  Zero and
+    infinity pass through, a NaN stays a NaN; a denormal gives its rounded
+    value or a zero of its sign (no network value is that small)."""
+    rng = np.random.default_rng(41)
+    normal = rng.integers(0x00800000, 0x7F800000, size=1 << 20, dtype=np.uint32)
+    sign = rng.integers(0, 2, size=normal.size, dtype=np.uint32) << 31
+    ties = (rng.integers(0x0080, 0x7F80, size=4096, dtype=np.uint32) << 16) | 0x8000
+    edge = np.array([0x7F7FFFFF, 0x7F7F8000, 0x7F7F7FFF, 0x3F808000, 0x3F818000, 0x00800000], np.uint32)
+    bits = np.concatenate([normal | sign, ties, ties | 0x80000000, edge, edge | 0x80000000])
+    x = bits.view(np.float32)
+    u = bits.astype(np.uint64)
+    want = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    assert_bitexact(probe(pkg, ctx, 7, x), want, "bf16 rounding")
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, 1.0, -2.0], np.float32)
+    assert_bitexact(probe(pkg, ctx, 7, sp), sp, "bf16 of exactly representable values")
+    assert np.isnan(probe(pkg, ctx, 7, np.array([np.nan], np.float32))).all()
+    den = rng.integers(1, 0x00800000, size=4096, dtype=np.uint32)
+    d = den.astype(np.uint64)
+    dwant = (((d + 0x7FFF + ((d >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    got = probe(pkg, ctx, 7, den.view(np.float32))
+    assert ((got.view(np.uint32) == dwant.view(np.uint32)) | (got == 0.0)).all()

@@ -61,9 +61,12 @@ def unpack(pkg, flat, shapes):
 
 
 def test_mfma_lane_layout(ctx, pkg):
+    """One v_mfma_f32_16x16x32_bf16 with the operand layout the kernels assume
+    (A[m][8g + j], B[8g + j][m], D[4g + j][m] for lane 16g + m) equals the
+    host product of the same bf16 matrices."""
     rng = np.random.default_rng(0)
-    A = rng.standard_normal((16, 16)).astype(np.float32)
-    B = rng.standard_normal((16, 16)).astype(np.float32)
+    A = rng.standard_normal((16, 32)).astype(np.float32)
+    B = rng.standard_normal((32, 16)).astype(np.float32)
     Ab, Af = bf16(A)
     Bb, Bf = bf16(B)
     D = np.zeros((16, 16), np.float32)
@@ -93,8 +96,8 @@ def test_forward_matches_oracle(ctx, pkg, M):
     assert err < 2e-2, err
 
 
-# 256 rows: 8 backward waves of one 32-row chunk each; 70000 rows: 1024 waves (the most the
-# backward launches), 140 of which take three chunks (the last one ragged) and the others two
+# 256 rows: 8 tiles of 32 rows and one gradient slot; 70000 rows: 2188 tiles, the last one ragged,
+# and 16 gradient slots (the most the weight-gradient GEMM uses)
 @pytest.mark.parametrize("B", [256, 70000])
 def test_gradients_match_oracle(ctx, pkg, M, B):
     ps = make_params(M, 3)
@@ -122,9 +125,10 @@ def test_gradients_match_oracle(ctx, pkg, M, B):
 
 
 def test_training_is_bit_reproducible(ctx, pkg, M):
-    """The backward pass sums gradients per wave and the Adam kernel adds the
-    waves in order (no float atomics), so the same steps from the same
-    parameters give bit-identical gradients, losses and parameters."""
+    """The weight-gradient GEMM sums each gradient slot in a fixed order and
+    the Adam kernel adds the slots in order (no float atomics), so the same
+    steps from the same parameters give bit-identical gradients, losses and
+    parameters."""
     ps = make_params(M, 9)
     flat = pack(pkg, ps)
     lib = pkg.load()
