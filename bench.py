@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "mlp"],
                     help="c1/c2: plain path tracer throughput; c4: 64 jobs batched vs one by one; "
                          "c5: learned vs plain time to relMSE <= 1e-3")
     ap.add_argument("--c5-clouds", type=int, default=400)
@@ -73,15 +73,20 @@ def log(msg):
 
 # the sources that define the path-tracing kernel's code
 PT_SOURCES = ("pt_kernel.h", "cr_device.h", "cr_internal.h", "kern_pt.hip")
+LBVH_SOURCES = ("kern_lbvh.hip", "cr_internal_ppb.h", "cr_device.h")
 
 
-def pt_sources_sha16():
+def sources_sha16(names):
     import hashlib
     h = hashlib.sha256()
-    for name in PT_SOURCES:
+    for name in names:
         with open(os.path.join(ROOT, "cloudrendering-vulkan_amd", "csrc", name), "rb") as f:
             h.update(name.encode() + b"\0" + f.read())
     return h.hexdigest()[:16]
+
+
+def pt_sources_sha16():
+    return sources_sha16(PT_SOURCES)
 
 
 def main():
@@ -96,6 +101,8 @@ def main():
     S = pkg.structs
     if args.workload == "c5":
         return bench_c5(args, pkg, cfg)
+    if args.workload == "mlp":
+        return bench_mlp(args, pkg)
     if args.workload == "c4":
         return bench_c4(args, pkg, cfg)
     if args.workload == "c3":
@@ -233,6 +240,57 @@ def rel_mse(img, ref, eps=1e-2):
     a = img[..., :3].astype(np.float64)
     r = ref[..., :3].astype(np.float64)
     return float(np.mean((a - r) ** 2 / (r * r + eps)))
+
+
+def bench_mlp(args, pkg):
+    """The learned network alone: --steps x 100 Adam steps of 8192 samples
+    and --steps inference calls on 1920 x 1080 rows, random features and
+    targets handed over from the host (cr_mlp_train_step / cr_mlp_forward).
+    The wall times include the host copies; the kernels' own times come from
+    a kernel trace of this command."""
+    import ctypes as C
+    import numpy as np
+    ctx = pkg.Context(0)
+    lib = ctx.lib
+    ctx.check(lib.cr_mlp_init(ctx.h, 3), "init")
+    B, rows = 8192, 1920 * 1080
+    rng = np.random.default_rng(1)
+    X = (rng.random((B, 224), dtype=np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    T = rng.random((B, 3), dtype=np.float32)
+    loss = C.c_float()
+    for _ in range(max(args.warmup, 1) * 20):
+        ctx.check(lib.cr_mlp_train_step(ctx.h, X.ctypes.data, T.ctypes.data, B, 1e-3, C.byref(loss), None), "step")
+    n_steps = max(args.steps, 1) * 100
+    t = time.time()
+    for _ in range(n_steps):
+        ctx.check(lib.cr_mlp_train_step(ctx.h, X.ctypes.data, T.ctypes.data, B, 1e-3, C.byref(loss), None), "step")
+    train_s = time.time() - t
+    XI = np.ascontiguousarray(np.tile(X, (rows // B + 1, 1))[:rows])
+    Y = np.zeros((rows, 3), np.float32)
+    ctx.check(lib.cr_mlp_forward(ctx.h, XI.ctypes.data, rows, Y.ctypes.data), "forward")
+    t = time.time()
+    for _ in range(max(args.steps, 1)):
+        ctx.check(lib.cr_mlp_forward(ctx.h, XI.ctypes.data, rows, Y.ctypes.data), "forward")
+    infer_s = (time.time() - t) / max(args.steps, 1)
+    flops = 2 * (224 * 128 + 3 * 128 * 128 + 128 * 16)
+    out = {
+        "metric": METRIC,
+        "value": round(n_steps / train_s, 1),
+        "unit": "Adam steps/s (8192 samples, host-driven: copies and loss readback included)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16 MFMA, fp32 accumulate",
+        "data": "synthetic",
+        "config": {"workload": "mlp: 224-128-128-128-128-3 network, 8192-sample Adam steps, 1920x1080-row inference"},
+        "steps": n_steps,
+        "final_loss": round(float(loss.value), 6),
+        "inference_rows": rows,
+        "inference_call_s_with_copies": round(infer_s, 4),
+        "inference_flop_per_row": flops,
+        "outputs_finite": bool(np.isfinite(Y).all()),
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def bench_c4(args, pkg, cfg):
@@ -387,13 +445,19 @@ def bench_c3(args, pkg, cfg):
     alg = 208.0 * nb
     achieved = alg / (build_ms * 1e-3) / 1e9
     # HBM bytes of one build, measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this workload
+    # (keyed to the kernel sources it was measured on: a stale file is not used)
     traffic = None
+    tsrc = "absent"
     tfile = os.path.join(ROOT, "profiles", "lbvh_c3_traffic.json")
     if os.path.exists(tfile):
         try:
             tf = json.load(open(tfile))
-            if tf.get("workload") == "c3" and tf.get("beams") == nb:
+            built = sources_sha16(LBVH_SOURCES)
+            if tf.get("workload") == "c3" and tf.get("beams") == nb and tf.get("kernel_sources_sha16") == built:
                 traffic = tf.get("hbm_bytes_per_build")
+                tsrc = f"profiles/lbvh_c3_traffic.json (kernel sources {built})"
+            else:
+                tsrc = f"stale: traffic measured on kernel sources {tf.get('kernel_sources_sha16')}, built sources {built}"
         except Exception:
             traffic = None
     out = {
@@ -412,7 +476,8 @@ def bench_c3(args, pkg, cfg):
         "stages_ms": stages,
         "trace_s": round(t_trace, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 4), "traffic": traffic, "kernel": "lbvh build (5 stages)",
+                     "frac": round(achieved / 8000.0, 4), "traffic": traffic, "traffic_source": tsrc,
+                     "kernel": "lbvh build (5 stages)",
                      "algorithmic_bytes_per_build": alg},
         "parity_vs_oracle": parity,
         "cpu_baseline": cpu,

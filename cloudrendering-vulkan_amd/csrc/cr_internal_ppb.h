@@ -26,6 +26,16 @@ struct TravNode {
 };
 static_assert(sizeof(TravNode) == 64, "TravNode is one 64-B line");
 
+// A beam in LBVH leaf order: the fields of cr_photon_beam the gather reads
+// (its offset along the parent beam is kept in a separate array).
+struct SortedBeam {
+    float startPos[3];
+    float radius;
+    float endPos[3];
+    uint32_t dataIdx;
+};
+static_assert(sizeof(SortedBeam) == 32, "SortedBeam is 32 B");
+
 hipError_t launch_morton(const cr_photon_beam* beams, uint32_t n, const BoxArg& box, uint32_t* keys, uint32_t* vals,
                          hipStream_t s);
 uint32_t sort_tiles(uint32_t n);
@@ -33,18 +43,19 @@ uint32_t sort_tiles(uint32_t n);
 uint32_t sort_counts_words(uint32_t n);
 hipError_t launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t* counts,
                              uint32_t n, hipStream_t s);
-// ends[i]: the other end j of internal node i's leaf range [min(i, j), max(i, j)]
-hipError_t launch_hierarchy(const uint32_t* keys, uint32_t n, uint32_t* parent, uint32_t* left, uint32_t* right,
-                            uint32_t* ends, hipStream_t s);
-hipError_t launch_leaves(const cr_photon_beam* beams, const uint32_t* perm, const uint32_t* keys, uint32_t n,
-                         cr_photon_beam* sorted, float* lo, float* hi, hipStream_t s);
-// exits: room for 2n node indices; exitCount and arrivals (n - 1 counters) zeroed beforehand
-hipError_t launch_refit(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, const uint32_t* ends,
-                        float* lo, float* hi, uint32_t* arrivals, uint32_t* exits, uint32_t* exitCount, hipStream_t s);
-hipError_t launch_pack(uint32_t n, const uint32_t* left, const uint32_t* right, const float* lo, const float* hi,
-                       TravNode* out, hipStream_t s);
-hipError_t launch_treenodes(uint32_t n, const uint32_t* parent, const uint32_t* left, const uint32_t* right, const float* lo,
-                            const float* hi, cr_tree_node* out, hipStream_t s);
+// meta[i] = (left, right, end, parent) of internal node i (end: the other end j
+// of its leaf range [min(i, j), max(i, j)]); leafParent[l]: the parent of leaf l
+hipError_t launch_hierarchy(const uint32_t* keys, uint32_t n, uint4* meta, uint32_t* leafParent, hipStream_t s);
+hipError_t launch_leaves(const cr_photon_beam* beams, const uint32_t* perm, uint32_t n, SortedBeam* sorted, float* sortedOff,
+                         float* lo, float* hi, hipStream_t s);
+// exits: refit_exit_words(n) words, the last one and arrivals (n - 1
+// counters) zeroed beforehand
+uint64_t refit_exit_words(uint32_t n);
+hipError_t launch_refit(uint32_t n, const uint4* meta, const uint32_t* leafParent, float* lo, float* hi, uint32_t* arrivals,
+                        uint32_t* exits, hipStream_t s);
+hipError_t launch_pack(uint32_t n, const uint4* meta, const float* lo, const float* hi, TravNode* out, hipStream_t s);
+hipError_t launch_treenodes(uint32_t n, const uint4* meta, const uint32_t* leafParent, const float* lo, const float* hi,
+                            cr_tree_node* out, hipStream_t s);
 
 // Beam producer: the scene a photon path sees.
 struct TraceArgs {
@@ -76,7 +87,8 @@ struct GatherArgs {
     cr_cloud_properties cloud;
     cr_parameters params;
     cr_shadow_volume_properties svp;
-    const cr_photon_beam* beams;     // sorted
+    const SortedBeam* beams;         // in leaf order
+    const float* beamOff;            //   their offsets along their parent beams
     const cr_photon_beam_data* data;
     const TravNode* nodes;
     const float* leafLo;             // box of node 0 when n == 1

@@ -20,12 +20,15 @@ struct Ppb {
     size_t beamsCap = 0;
     cr_photon_beam_data* data = nullptr;
     size_t dataCap = 0;
-    cr_photon_beam* sorted = nullptr;
+    SortedBeam* sorted = nullptr;
     size_t sortedCap = 0;
+    float* sortedOff = nullptr;
+    size_t sortedOffCap = 0;
     uint32_t *keys = nullptr, *vals = nullptr, *keysTmp = nullptr, *valsTmp = nullptr, *counts = nullptr;
     size_t keysCap = 0, valsCap = 0, keysTmpCap = 0, valsTmpCap = 0, countsCap = 0;
-    uint32_t *parent = nullptr, *left = nullptr, *right = nullptr, *arrivals = nullptr, *ends = nullptr, *exits = nullptr;
-    size_t parentCap = 0, leftCap = 0, rightCap = 0, arrivalsCap = 0, endsCap = 0, exitsCap = 0;
+    uint4* meta = nullptr;      // per internal node: left, right, range end, parent
+    uint32_t *leafParent = nullptr, *arrivals = nullptr, *exits = nullptr;
+    size_t metaCap = 0, leafParentCap = 0, arrivalsCap = 0, exitsCap = 0;
     float *lo = nullptr, *hi = nullptr;
     size_t loCap = 0, hiCap = 0;
     TravNode* trav = nullptr;
@@ -47,8 +50,8 @@ void ppb_free(cr_context* ctx)
 {
     Ppb* p = ctx->ppb;
     if (!p) return;
-    void* bufs[] = {p->beams, p->data, p->sorted, p->keys, p->vals, p->keysTmp, p->valsTmp, p->counts, p->parent,
-                    p->left, p->right, p->arrivals, p->ends, p->exits, p->lo, p->hi, p->trav, p->gcounters, p->overflow};
+    void* bufs[] = {p->beams, p->data, p->sorted, p->sortedOff, p->keys, p->vals, p->keysTmp, p->valsTmp, p->counts, p->meta,
+                    p->leafParent, p->arrivals, p->exits, p->lo, p->hi, p->trav, p->gcounters, p->overflow};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (hipEvent_t e : p->ev)
@@ -89,19 +92,18 @@ static const uint32_t MAX_BEAMS = 1u << 28;
 static cr_status lbvh_buffers(cr_context* ctx, Ppb* p, uint32_t n)
 {
     const size_t nodes = 2 * (size_t)n - 1;
-    CR_TRY(ensure_buffer(ctx, (void**)&p->sorted, &p->sortedCap, (size_t)n * sizeof(cr_photon_beam), "sorted beams"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->sorted, &p->sortedCap, (size_t)n * sizeof(SortedBeam), "sorted beams"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->sortedOff, &p->sortedOffCap, (size_t)n * 4, "sorted beam offsets"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->keys, &p->keysCap, (size_t)n * 4, "keys"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->vals, &p->valsCap, (size_t)n * 4, "vals"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->keysTmp, &p->keysTmpCap, (size_t)n * 4, "keys tmp"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->valsTmp, &p->valsTmpCap, (size_t)n * 4, "vals tmp"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->counts, &p->countsCap, (size_t)sort_counts_words(n) * 4, "histograms"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->parent, &p->parentCap, nodes * 4, "parent"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->left, &p->leftCap, (size_t)n * 4, "left"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->right, &p->rightCap, (size_t)n * 4, "right"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->meta, &p->metaCap, (size_t)n * 16, "node records"));
+    CR_TRY(ensure_buffer(ctx, (void**)&p->leafParent, &p->leafParentCap, (size_t)n * 4, "leaf parents"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->arrivals, &p->arrivalsCap, (size_t)n * 4, "arrivals"));
-    CR_TRY(ensure_buffer(ctx, (void**)&p->ends, &p->endsCap, (size_t)n * 4, "range ends"));
-    // refit exits (at most two per internal node) and their count at the end
-    CR_TRY(ensure_buffer(ctx, (void**)&p->exits, &p->exitsCap, (2 * (size_t)n + 1) * 4, "refit exits"));
+    // refit exit lists (each at most two entries per internal node) and their counts at the end
+    CR_TRY(ensure_buffer(ctx, (void**)&p->exits, &p->exitsCap, (size_t)refit_exit_words(n) * 4, "refit exits"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->lo, &p->loCap, nodes * 12, "box lo"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->hi, &p->hiCap, nodes * 12, "box hi"));
     CR_TRY(ensure_buffer(ctx, (void**)&p->trav, &p->travCap, (size_t)n * sizeof(TravNode), "traversal nodes"));
@@ -262,24 +264,22 @@ cr_status cr_lbvh_build(cr_context* ctx)
     }
     hipStream_t s = ctx->stream;
     CR_HIP(ctx, zero_async(p->arrivals, (size_t)n * 4, s), "zero arrivals");
-    CR_HIP(ctx, zero_async(p->exits + 2 * (size_t)n, 4, s), "zero exit count");
-    if (n == 1) CR_HIP(ctx, zero_async(p->parent, 4, s), "zero parent");
+    CR_HIP(ctx, zero_async(p->exits + refit_exit_words(n) - 1, 4, s), "zero exit count");
     CR_HIP(ctx, hipEventRecord(p->ev[0], s), "event");
     CR_HIP(ctx, launch_morton(p->beams, n, box, p->keys, p->vals, s), "morton kernel");
     CR_HIP(ctx, hipEventRecord(p->ev[1], s), "event");
     CR_HIP(ctx, launch_radix_sort(p->keys, p->vals, p->keysTmp, p->valsTmp, p->counts, n, s), "radix sort");
     CR_HIP(ctx, hipEventRecord(p->ev[2], s), "event");
-    CR_HIP(ctx, launch_hierarchy(p->keys, n, p->parent, p->left, p->right, p->ends, s), "hierarchy kernel");
+    CR_HIP(ctx, launch_hierarchy(p->keys, n, p->meta, p->leafParent, s), "hierarchy kernel");
     CR_HIP(ctx, hipEventRecord(p->ev[3], s), "event");
-    CR_HIP(ctx, launch_leaves(p->beams, p->vals, p->keys, n, p->sorted, p->lo, p->hi, s), "leaf kernel");
-    CR_HIP(ctx, launch_refit(n, p->parent, p->left, p->right, p->ends, p->lo, p->hi, p->arrivals, p->exits,
-                             p->exits + 2 * (size_t)n, s), "refit kernels");
+    CR_HIP(ctx, launch_leaves(p->beams, p->vals, n, p->sorted, p->sortedOff, p->lo, p->hi, s), "leaf kernel");
+    CR_HIP(ctx, launch_refit(n, p->meta, p->leafParent, p->lo, p->hi, p->arrivals, p->exits, s), "refit kernels");
     CR_HIP(ctx, hipEventRecord(p->ev[4], s), "event");
-    CR_HIP(ctx, launch_pack(n, p->left, p->right, p->lo, p->hi, p->trav, s), "pack kernel");
+    CR_HIP(ctx, launch_pack(n, p->meta, p->lo, p->hi, p->trav, s), "pack kernel");
     CR_HIP(ctx, hipEventRecord(p->ev[5], s), "event");
     if (n == 1) {
         uint32_t none = 0xFFFFFFFFu;
-        CR_HIP(ctx, hipMemcpyAsync(p->parent, &none, 4, hipMemcpyHostToDevice, s), "root parent");
+        CR_HIP(ctx, hipMemcpyAsync(p->leafParent, &none, 4, hipMemcpyHostToDevice, s), "root parent");
     }
     CR_HIP(ctx, hipStreamSynchronize(s), "sync");
     p->built = true;
@@ -314,7 +314,7 @@ cr_status cr_lbvh_download(cr_context* ctx, uint32_t* morton_sorted, uint32_t* p
         cr_tree_node* tmp = nullptr;
         const size_t bytes = (2 * (size_t)n - 1) * sizeof(cr_tree_node);
         CR_HIP(ctx, hipMalloc((void**)&tmp, bytes), "hipMalloc nodes");
-        hipError_t e = launch_treenodes(n, p->parent, p->left, p->right, p->lo, p->hi, tmp, s);
+        hipError_t e = launch_treenodes(n, p->meta, p->leafParent, p->lo, p->hi, tmp, s);
         if (e == hipSuccess) e = hipMemcpyAsync(nodes, tmp, bytes, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         hipFree(tmp);
@@ -350,6 +350,7 @@ cr_status cr_ppb_gather(cr_context* ctx, const cr_push_constants* frames, uint32
     a.params = ctx->params;
     a.svp = ctx->svp;
     a.beams = p->sorted;
+    a.beamOff = p->sortedOff;
     a.data = p->data;
     a.nodes = p->trav;
     a.leafLo = p->lo;

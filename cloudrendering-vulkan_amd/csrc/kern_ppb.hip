@@ -213,7 +213,7 @@ __device__ __forceinline__ bool hit_box(f3 ro, f3 rd, f3 m, const float* cen3, c
 template <typename T, int LAYOUT>
 __device__ __forceinline__ void beam_radiance(const cr::GatherArgs& a, uint32_t bi, f3 ro, f3 rd, f4& acc)
 {
-    const cr_photon_beam B = a.beams[bi];
+    const cr::SortedBeam B = a.beams[bi];
     const f3 st = ld3(B.startPos), en = ld3(B.endPos);
     {
         // Skip a beam whose line passes the camera ray's line farther than its
@@ -252,7 +252,7 @@ __device__ __forceinline__ void beam_radiance(const cr::GatherArgs& a, uint32_t 
     svg.data = a.sv; svg.nx = a.svx; svg.ny = a.svy; svg.nz = a.svz; svg.nbx = 0; svg.nby = 0;
     const float tcam = trilinear<float, LAYOUT_DENSE>(svg, sv_norm(a.svp, x));
     const cr_photon_beam_data& D = a.data[B.dataIdx];
-    const float soff = a.refCompat ? s : (__uint_as_float(B._padding_beam[0]) + s);
+    const float soff = a.refCompat ? s : (a.beamOff[bi] + s);
     uint32_t cnt = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++)
