@@ -19,8 +19,10 @@
  *             low part (Cody & Waite's reduction); <= 1.3 ULP.
  *  o_expf   : x = k ln2 + r, |r| <= ln2/2 (Cody-Waite reduction), Taylor
  *             series of e^r to r^8, scaled by 2^k.
- *  o_sincosf: quadrant reduction by pi/2 in three parts (Cody-Waite),
- *             Taylor series on [-pi/4, pi/4] (sin to r^11, cos to r^12);
+ *  o_sincosf: quadrant reduction by pi/2 in three parts (Cody-Waite; the
+ *             split 1.5703125 + 4.837512969970703e-4 + 7.54978995489188e-8
+ *             is twice the pi/4 split of Cephes' sinf, DP1..DP3), Taylor
+ *             series on [-pi/4, pi/4] (sin to r^11, cos to r^12);
  *             defined for |x| <= 2^17, NaN beyond (the three-part reduction
  *             loses accuracy there, and the quadrant would need a float to
  *             int conversion that C leaves undefined for huge values).
