@@ -211,7 +211,7 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
     const bool shadow = L.mode == MODE_SHADOW;
     const f3 dir = shadow ? sdir : L.dir;
     if (TRACK == CR_TRACK_BRICK_DDA) {
-        float maj = 0.0f;
+        float maj = 0.0f, sbHit = 0.0f;
         // Empty bricks (skip entry <= 0) are crossed with no lookup.  Every
         // brick's entry is fetched, two at a time: with the loads overlapped,
         // that is cheaper than testing whether the walk is still inside a cube
@@ -249,7 +249,7 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
                 float seg = tc - L.t;
                 c.brickVisits++;
                 if (sb > 0.0f && L.tau < sb * seg) {
-                    L.t = L.t + fdiv(L.tau, sb);
+                    sbHit = sb;
                     collided = true;
                     break;
                 }
@@ -259,23 +259,28 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
                 if (!dda_advance(a, L)) return EV_EXIT;
             }
         }
-        // the loop leaves only from a brick whose entry it fetched (sb > 0)
+        // the loop leaves only from a brick whose entry it fetched (sb > 0);
+        // the division is made once here, not at each unrolled visit
+        L.t = L.t + fdiv(L.tau, sbHit);
         f3 cur = add(L.pos, mul(dir, L.t));
         float e = sample_cloud<T, LAYOUT>(a, cur);
         c.lookups++;
+        // shadow and primary lanes of a wave share the division and the next
+        // free-flight draw (one copy of each instead of one per branch)
+        const float q = fdiv(e, maj);
+        uint32_t ev = EV_NONE;
         if (shadow) {
-            L.T = L.T * (1.0f - fdiv(e, maj));
-            if (L.T == 0.0f) return EV_EXIT;
-            L.tau = -crd_logf(xs_float(L.rng));
-            return EV_NONE;
+            L.T = L.T * (1.0f - q);
+            if (L.T == 0.0f) ev = EV_EXIT;
+        } else {
+            float xi = xs_float(L.rng);
+            if (xi < q) {
+                L.pos = cur;
+                ev = EV_COLLIDE;
+            }
         }
-        float xi = xs_float(L.rng);
-        if (xi < fdiv(e, maj)) {
-            L.pos = cur;
-            return EV_COLLIDE;
-        }
-        L.tau = -crd_logf(xs_float(L.rng));
-        return EV_NONE;
+        if (ev == EV_NONE) L.tau = -crd_logf(xs_float(L.rng));
+        return ev;
     }
     float zeta = xs_float(L.rng);
     L.t = L.t + fdiv(-crd_logf(zeta), a.sigmaBar);
@@ -283,13 +288,14 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
     f3 cur = add(L.pos, mul(dir, L.t));
     float e = sample_cloud<T, LAYOUT>(a, cur);
     c.lookups++;
+    const float q = fdiv(e, a.cloud.maxExtinction);
     if (shadow) {
-        L.T = L.T * (1.0f - fdiv(e, a.cloud.maxExtinction));
+        L.T = L.T * (1.0f - q);
         if (L.T == 0.0f) return EV_EXIT;
         return EV_NONE;
     }
     float xi = xs_float(L.rng);
-    if (xi < fdiv(e, a.cloud.maxExtinction)) {
+    if (xi < q) {
         L.pos = cur;
         return EV_COLLIDE;
     }
@@ -494,6 +500,10 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
         while (ev != EV_NONE || needSample) {
             bool doScatter = false;  // sample a new direction along which to continue
             uint32_t seg = 0;        // start a segment: 1 along L.dir, 2 toward the sun
+            // The ends of a sample share one copy of the background and of the
+            // blend: sky = the path left the box (camera rays that miss it take
+            // the background as their radiance), fin = the sample is complete.
+            bool sky = false, missed = false, fin = false;
             if (needSample) {
                 needSample = false;
                 if (L.f >= (KMODE == K_CONTINUE ? a.spp : a.nFrames)) {
@@ -548,22 +558,15 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 float tmax = 0.0f, tmin = 0.0f;
                 if (!intersect_box(B0(a), B1(a), L.pos, L.dir, tmin, tmax) || tmax < 0.0f ||
                     a.cloud.densityScaling <= 0.0f) {
-                    L.res = background(a.ext, L.dir);
-                    finish_sample<KMODE>(a, L);
-                    L.f++;
-                    needSample = true;
-                    continue;
+                    sky = true;
+                    missed = true;
+                } else {
+                    L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
+                    seg = 1;
                 }
-                L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
-                seg = 1;
             } else if (ev == EV_EXIT && L.mode == MODE_PRIMARY) {
                 // escape: PathTracer.comp:399-403
-                add_scaled(L.res, background(a.ext, L.dir), L.thr);
-                finish_sample<KMODE>(a, L);
-                L.f++;
-                ev = EV_NONE;
-                needSample = true;
-                continue;
+                sky = true;
             } else if (ev == EV_EXIT && L.mode == MODE_SHADOW) {
                 // shadow ray done: add NEE, then scatter
                 add_scalar(L.res, ((L.ipdf * L.T) * L.w));
@@ -583,34 +586,47 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 }
                 L.bounces++;
                 if (L.bounces > cap || L.bounces > COLLISION_CEILING) {
-                    finish_sample<KMODE>(a, L);
-                    L.f++;
-                    ev = EV_NONE;
-                    needSample = true;
-                    continue;
-                }
-                c.vertices++;
-                const float pdf = phase_pdf(a.params, neg(L.dir), ldir);
-                const float w = L.thr * albedo;
-                if (NEE == CR_NEE_RATIO_TRACKING) {
-                    // a shadow ray that misses the box keeps T = 1 and draws nothing
-                    L.ipdf = (1.0f * I) * pdf;
-                    L.w = w;
-                    L.T = 1.0f;
-                    L.mode = MODE_SHADOW;
-                    seg = 2;
+                    fin = true;
                 } else {
-                    float Tv;
-                    if (NEE == CR_NEE_SHADOW_VOLUME) {
-                        Tv = sample_sv(a, L.pos);
-                        c.svLookups++;
+                    c.vertices++;
+                    const float pdf = phase_pdf(a.params, neg(L.dir), ldir);
+                    const float w = L.thr * albedo;
+                    if (NEE == CR_NEE_RATIO_TRACKING) {
+                        // a shadow ray that misses the box keeps T = 1 and draws nothing
+                        L.ipdf = (1.0f * I) * pdf;
+                        L.w = w;
+                        L.T = 1.0f;
+                        L.mode = MODE_SHADOW;
+                        seg = 2;
                     } else {
-                        Tv = 0.0f;
+                        float Tv;
+                        if (NEE == CR_NEE_SHADOW_VOLUME) {
+                            Tv = sample_sv(a, L.pos);
+                            c.svLookups++;
+                        } else {
+                            Tv = 0.0f;
+                        }
+                        add_scalar(L.res, (((1.0f * I) * pdf) * Tv) * w);
+                        L.thr = w;
+                        doScatter = true;
                     }
-                    add_scalar(L.res, (((1.0f * I) * pdf) * Tv) * w);
-                    L.thr = w;
-                    doScatter = true;
                 }
+            }
+            if (sky) {
+                const f4 bg = background(a.ext, L.dir);
+                if (missed) {
+                    L.res = bg;
+                } else {
+                    add_scaled(L.res, bg, L.thr);
+                }
+                fin = true;
+            }
+            if (fin) {
+                finish_sample<KMODE>(a, L);
+                L.f++;
+                ev = EV_NONE;
+                needSample = true;
+                continue;
             }
             if (doScatter) {
                 L.dir = scatter(a.params, L.dir, L.rng);
