@@ -182,14 +182,13 @@ __device__ __forceinline__ float2 brick_skip(const cr::PTArgs& a, uint32_t brick
     return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(a.skip) + (brick << 3));
 }
 
-// Start a tracking segment from L.pos along `dir` (primary or shadow).
-// Returns false when the segment misses the box (no RNG draw is made then,
-// as findScatterPoint returns before tracking, PathTracer.comp:321-324).
+// Start a tracking segment from L.pos along `dir` (primary or shadow); invd =
+// 1 / dir.  Returns false when the segment misses the box (no RNG draw is made
+// then, as findScatterPoint returns before tracking, PathTracer.comp:321-324).
 template <int TRACK>
-__device__ __forceinline__ bool begin_segment(const cr::PTArgs& a, Lane& L, f3 dir)
+__device__ __forceinline__ bool begin_segment(const cr::PTArgs& a, Lane& L, f3 dir, f3 invd)
 {
     float tmax = 0.0f, tmin = 0.0f;
-    const f3 invd = recip(dir);
     if (!intersect_box_inv(B0(a), B1(a), L.pos, invd, tmin, tmax)) return false;
     f3 exitp = add(L.pos, mul(dir, tmax));
     L.dist = length(sub(L.pos, exitp));
@@ -504,6 +503,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
             // blend: sky = the path left the box (camera rays that miss it take
             // the background as their radiance), fin = the sample is complete.
             bool sky = false, missed = false, fin = false;
+            bool cam = false;  // a camera ray: its box test comes after the shared reciprocal below
             if (needSample) {
                 needSample = false;
                 if (L.f >= (KMODE == K_CONTINUE ? a.spp : a.nFrames)) {
@@ -555,15 +555,8 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 L.thr = 1.0f;
                 L.bounces = 0;
                 L.mode = MODE_PRIMARY;
-                float tmax = 0.0f, tmin = 0.0f;
-                if (!intersect_box(B0(a), B1(a), L.pos, L.dir, tmin, tmax) || tmax < 0.0f ||
-                    a.cloud.densityScaling <= 0.0f) {
-                    sky = true;
-                    missed = true;
-                } else {
-                    L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
-                    seg = 1;
-                }
+                cam = true;
+                seg = 1;
             } else if (ev == EV_EXIT && L.mode == MODE_PRIMARY) {
                 // escape: PathTracer.comp:399-403
                 sky = true;
@@ -612,6 +605,24 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                     }
                 }
             }
+            if (doScatter) {
+                L.dir = scatter(a.params, L.dir, L.rng);
+                seg = 1;
+            }
+            // the direction of the segment each lane starts, and its reciprocal,
+            // formed once for the camera rays' box test and the segment set-up
+            const f3 sd = seg == 2 ? sdir : L.dir;
+            const f3 invd = recip(sd);
+            if (cam) {
+                float tmax = 0.0f, tmin = 0.0f;
+                if (!intersect_box_inv(B0(a), B1(a), L.pos, invd, tmin, tmax) || tmax < 0.0f ||
+                    a.cloud.densityScaling <= 0.0f) {
+                    sky = true;
+                    missed = true;
+                } else {
+                    L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
+                }
+            }
             if (sky) {
                 const f4 bg = background(a.ext, L.dir);
                 if (missed) {
@@ -628,12 +639,8 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 needSample = true;
                 continue;
             }
-            if (doScatter) {
-                L.dir = scatter(a.params, L.dir, L.rng);
-                seg = 1;
-            }
             // seg != 0 here: every branch that falls through starts a segment
-            ev = begin_segment<TRACK>(a, L, seg == 2 ? sdir : L.dir) ? EV_NONE : EV_EXIT;
+            ev = begin_segment<TRACK>(a, L, sd, invd) ? EV_NONE : EV_EXIT;
         }
     }
 
