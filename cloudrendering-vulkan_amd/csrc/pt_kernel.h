@@ -58,6 +58,10 @@ struct Lane {
     int32_t px, py;
     uint32_t id;
     uint32_t f;
+    // the pixel's camera ray, the same for every frame: its entry point into
+    // the box (or the camera position when it misses the box) and direction
+    f3 camPos, camDir;
+    bool camMiss;
     f4 acc;
     uint32_t rng;
     f4 res;
@@ -462,6 +466,13 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                             float4 v = J.image[(size_t)y * J.W + x];
                             L.acc.x = v.x; L.acc.y = v.y; L.acc.z = v.z; L.acc.w = v.w;
                         }
+                        f3 cp, cd;
+                        camera_ray(J.cam, (float)(x - J.cam.halfWidth), (float)(y - J.cam.halfHeight), cp, cd);
+                        float tmax = 0.0f, tmin = 0.0f;
+                        L.camMiss = !intersect_box(B0(J), B1(J), cp, cd, tmin, tmax) || tmax < 0.0f ||
+                                    J.cloud.densityScaling <= 0.0f;
+                        L.camPos = L.camMiss ? cp : add(cp, mul(cd, tmax < tmin ? tmax : tmin));
+                        L.camDir = cd;
                         L.f = 0;
                         L.cost0 = c.lookups + c.brickVisits;
                         state = ST_ACTIVE;
@@ -503,7 +514,6 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
             // blend: sky = the path left the box (camera rays that miss it take
             // the background as their radiance), fin = the sample is complete.
             bool sky = false, missed = false, fin = false;
-            bool cam = false;  // a camera ray: its box test comes after the shared reciprocal below
             if (needSample) {
                 needSample = false;
                 if (L.f >= (KMODE == K_CONTINUE ? a.spp : a.nFrames)) {
@@ -545,18 +555,24 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                     ev = EV_COLLIDE;
                     continue;
                 }
-                // start frame f: PathTracer.comp:368-392
+                // start frame f: PathTracer.comp:368-392 (the camera ray and its box
+                // test were made when the lane took the pixel)
                 const int32_t seed = a.frames[2 * L.f];
                 xs_init(L.rng, (uint32_t)seed * L.id);
-                camera_ray(a.cam, (float)(L.px - a.cam.halfWidth), (float)(L.py - a.cam.halfHeight), L.pos, L.dir);
+                L.pos = L.camPos;
+                L.dir = L.camDir;
                 c.paths++;
                 c.vertices++;
                 L.res.x = 0.0f; L.res.y = 0.0f; L.res.z = 0.0f; L.res.w = 0.0f;
                 L.thr = 1.0f;
                 L.bounces = 0;
                 L.mode = MODE_PRIMARY;
-                cam = true;
-                seg = 1;
+                if (L.camMiss) {
+                    sky = true;
+                    missed = true;
+                } else {
+                    seg = 1;
+                }
             } else if (ev == EV_EXIT && L.mode == MODE_PRIMARY) {
                 // escape: PathTracer.comp:399-403
                 sky = true;
@@ -609,20 +625,6 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 L.dir = scatter(a.params, L.dir, L.rng);
                 seg = 1;
             }
-            // the direction of the segment each lane starts, and its reciprocal,
-            // formed once for the camera rays' box test and the segment set-up
-            const f3 sd = seg == 2 ? sdir : L.dir;
-            const f3 invd = recip(sd);
-            if (cam) {
-                float tmax = 0.0f, tmin = 0.0f;
-                if (!intersect_box_inv(B0(a), B1(a), L.pos, invd, tmin, tmax) || tmax < 0.0f ||
-                    a.cloud.densityScaling <= 0.0f) {
-                    sky = true;
-                    missed = true;
-                } else {
-                    L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
-                }
-            }
             if (sky) {
                 const f4 bg = background(a.ext, L.dir);
                 if (missed) {
@@ -640,7 +642,8 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 continue;
             }
             // seg != 0 here: every branch that falls through starts a segment
-            ev = begin_segment<TRACK>(a, L, sd, invd) ? EV_NONE : EV_EXIT;
+            const f3 sd = seg == 2 ? sdir : L.dir;
+            ev = begin_segment<TRACK>(a, L, sd, recip(sd)) ? EV_NONE : EV_EXIT;
         }
     }
 
