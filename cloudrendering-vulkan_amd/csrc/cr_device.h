@@ -84,18 +84,13 @@ __device__ __forceinline__ f3 ld3(const float* p) { return mk(p[0], p[1], p[2]);
 
 __device__ __forceinline__ float fdiv(float a, float b) { return __fdiv_rn(a, b); }
 
-__device__ __forceinline__ float crd_logf(float x)
+// log x for a normal, finite x > 0 (scaled by 2^e0 before the call): the
+// core of crd_logf without its special cases.  xs_float's values, in
+// [2^-32, 1], are such x, so the free-flight draws call this directly.
+__device__ __forceinline__ float crd_log_normal(float x, int32_t e0 = 0)
 {
-    if (x == 0.0f) return -__builtin_inff();
-    if (!(x > 0.0f)) return __builtin_nanf("");
-    if (x == __builtin_inff()) return x;
-    int32_t e = 0;
-    uint32_t b = __float_as_uint(x);
-    if (b < 0x00800000u) {
-        x = x * 8388608.0f;
-        b = __float_as_uint(x);
-        e = -23;
-    }
+    int32_t e = e0;
+    const uint32_t b = __float_as_uint(x);
     e += (int32_t)((b >> 23) & 0xffu) - 127;
     float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
     if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
@@ -112,6 +107,15 @@ __device__ __forceinline__ float crd_logf(float x)
     float logm = (f - 0.5f * z) + (z * f) * p;
     float fe = (float)e;
     return fe * 0.693145751953125f + (logm + fe * 1.42860682030941723212e-6f);
+}
+
+__device__ __forceinline__ float crd_logf(float x)
+{
+    if (x == 0.0f) return -__builtin_inff();
+    if (!(x > 0.0f)) return __builtin_nanf("");
+    if (x == __builtin_inff()) return x;
+    if (__float_as_uint(x) < 0x00800000u) return crd_log_normal(x * 8388608.0f, -23);
+    return crd_log_normal(x);
 }
 
 __device__ __forceinline__ float crd_expf(float x)

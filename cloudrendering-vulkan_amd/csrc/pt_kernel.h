@@ -199,7 +199,7 @@ __device__ __forceinline__ bool begin_segment(const cr::PTArgs& a, Lane& L, f3 d
     L.t = 0.0f;
     if (TRACK == CR_TRACK_BRICK_DDA) {
         dda_init(a, L.pos, dir, invd, L);
-        L.tau = -crd_logf(xs_float(L.rng));
+        L.tau = -crd_log_normal(xs_float(L.rng));
     }
     return true;
 }
@@ -282,11 +282,11 @@ __device__ __forceinline__ uint32_t track_step(const cr::PTArgs& a, Lane& L, f3 
                 ev = EV_COLLIDE;
             }
         }
-        if (ev == EV_NONE) L.tau = -crd_logf(xs_float(L.rng));
+        if (ev == EV_NONE) L.tau = -crd_log_normal(xs_float(L.rng));
         return ev;
     }
     float zeta = xs_float(L.rng);
-    L.t = L.t + fdiv(-crd_logf(zeta), a.sigmaBar);
+    L.t = L.t + fdiv(-crd_log_normal(zeta), a.sigmaBar);
     if (!(L.t < L.dist)) return EV_EXIT;
     f3 cur = add(L.pos, mul(dir, L.t));
     float e = sample_cloud<T, LAYOUT>(a, cur);
