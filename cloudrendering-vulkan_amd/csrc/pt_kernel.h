@@ -62,6 +62,10 @@ struct Lane {
     // the box (or the camera position when it misses the box) and direction
     f3 camPos, camDir;
     bool camMiss;
+    // (seed, frameCount) of the next frame, loaded one frame ahead so that
+    // starting and blending a frame does not wait on memory
+    int32_t seedNext, countNext;
+    uint32_t count;  // frameCount of the current frame
     f4 acc;
     uint32_t rng;
     f4 res;
@@ -324,7 +328,7 @@ __device__ __forceinline__ void add_scalar(f4& r, float s)
 // Blend the finished sample into the running mean (PathTracer.comp:420-424).
 __device__ __forceinline__ void blend(const cr::PTArgs& a, Lane& L)
 {
-    const uint32_t fc = (uint32_t)a.frames[2 * L.f + 1];
+    const uint32_t fc = L.count;
     const float n = (float)fc;
     const float d = (float)(fc + 1u);
     L.acc.x = fdiv(L.res.x + L.acc.x * n, d);
@@ -473,6 +477,10 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                                     J.cloud.densityScaling <= 0.0f;
                         L.camPos = L.camMiss ? cp : add(cp, mul(cd, tmax < tmin ? tmax : tmin));
                         L.camDir = cd;
+                        if (J.nFrames > 0u) {
+                            L.seedNext = J.frames[0];
+                            L.countNext = J.frames[1];
+                        }
                         L.f = 0;
                         L.cost0 = c.lookups + c.brickVisits;
                         state = ST_ACTIVE;
@@ -557,7 +565,12 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 }
                 // start frame f: PathTracer.comp:368-392 (the camera ray and its box
                 // test were made when the lane took the pixel)
-                const int32_t seed = a.frames[2 * L.f];
+                const int32_t seed = L.seedNext;
+                L.count = (uint32_t)L.countNext;
+                if (L.f + 1u < a.nFrames) {
+                    L.seedNext = a.frames[2 * (L.f + 1u)];
+                    L.countNext = a.frames[2 * (L.f + 1u) + 1];
+                }
                 xs_init(L.rng, (uint32_t)seed * L.id);
                 L.pos = L.camPos;
                 L.dir = L.camDir;
