@@ -339,8 +339,19 @@ __global__ void __launch_bounds__(GATHER_THREADS) gather_kernel(cr::GatherArgs a
                     const bool hl = hit_box(ro, rd, m, nd.cen[0], nd.rad[0]);
                     const bool hr = hit_box(ro, rd, m, nd.cen[1], nd.rad[1]);
                     const bool ll = (nd.leafMask & 1u) != 0, lr = (nd.leafMask & 2u) != 0;
-                    if (hl && ll) { beam_radiance<T, LAYOUT>(a, nd.child[0] - inner, ro, rd, L); tested++; }
-                    if (hr && lr) { beam_radiance<T, LAYOUT>(a, nd.child[1] - inner, ro, rd, L); tested++; }
+                    // The node's hit leaves, left before right, through one copy
+                    // of the beam test: a wave whose lanes hit only left leaves
+                    // and only right leaves runs it once, not once per side.
+                    uint32_t first = (hl && ll) ? nd.child[0] : ((hr && lr) ? nd.child[1] : NONE);
+                    const uint32_t second = (hl && ll && hr && lr) ? nd.child[1] : NONE;
+#pragma unroll 1
+                    for (int k = 0; k < 2; k++) {
+                        if (first != NONE) {
+                            beam_radiance<T, LAYOUT>(a, first - inner, ro, rd, L);
+                            tested++;
+                        }
+                        first = second;
+                    }
                     const bool tl = hl && !ll, tr = hr && !lr;
                     if (!tl && !tr) {
                         sp--;
