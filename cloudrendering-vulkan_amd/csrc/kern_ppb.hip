@@ -194,9 +194,10 @@ __device__ __forceinline__ bool outside_slab(float ro, float d, float cen, float
 }
 
 // intersectBox (PPB_PE.comp:282-300) with rad = half the box size; the box as
-// center and half extent, m = 1 / rd formed once per ray; plus the test of
-// the axes the ray is parallel to.
-__device__ __forceinline__ bool hit_box(f3 ro, f3 rd, f3 m, const float* cen3, const float* rad3)
+// center and half extent, m = 1 / rd formed once per ray; plus, for a ray
+// with a zero direction component (par), the test of the axes it is parallel
+// to (outside_slab is false on every axis of any other ray).
+__device__ __forceinline__ bool hit_box(f3 ro, f3 rd, f3 m, bool par, const float* cen3, const float* rad3)
 {
     const f3 cen = mk(cen3[0], cen3[1], cen3[2]);
     const f3 rad = mk(rad3[0], rad3[1], rad3[2]);
@@ -204,8 +205,10 @@ __device__ __forceinline__ bool hit_box(f3 ro, f3 rd, f3 m, const float* cen3, c
     const f3 k = mk(fabsf(m.x) * rad.x, fabsf(m.y) * rad.y, fabsf(m.z) * rad.z);
     const float tN = fmaxf(fmaxf(-n.x - k.x, -n.y - k.y), -n.z - k.z);
     const float tF = fminf(fminf(-n.x + k.x, -n.y + k.y), -n.z + k.z);
-    const bool parallelMiss = outside_slab(ro.x, rd.x, cen.x, rad.x) || outside_slab(ro.y, rd.y, cen.y, rad.y) ||
-                              outside_slab(ro.z, rd.z, cen.z, rad.z);
+    bool parallelMiss = false;
+    if (par)
+        parallelMiss = outside_slab(ro.x, rd.x, cen.x, rad.x) || outside_slab(ro.y, rd.y, cen.y, rad.y) ||
+                       outside_slab(ro.z, rd.z, cen.z, rad.z);
     return !(tN > tF || tF < 0.0f || parallelMiss);
 }
 
@@ -311,13 +314,14 @@ __global__ void __launch_bounds__(GATHER_THREADS) gather_kernel(cr::GatherArgs a
             f4 L;
             L.x = 0.0f; L.y = 0.0f; L.z = 0.0f; L.w = 0.0f;
             const f3 m = mk(fdiv(1.0f, rd.x), fdiv(1.0f, rd.y), fdiv(1.0f, rd.z));
+            const bool par = rd.x == 0.0f || rd.y == 0.0f || rd.z == 0.0f;
             if (a.n == 1) {
                 // (lo + hi) / 2: halving is exact, so * 0.5 is the same value
                 const float cen[3] = {(a.leafLo[0] + a.leafHi[0]) * 0.5f, (a.leafLo[1] + a.leafHi[1]) * 0.5f,
                                       (a.leafLo[2] + a.leafHi[2]) * 0.5f};
                 const float rad[3] = {(a.leafHi[0] - a.leafLo[0]) * 0.5f, (a.leafHi[1] - a.leafLo[1]) * 0.5f,
                                       (a.leafHi[2] - a.leafLo[2]) * 0.5f};
-                if (hit_box(ro, rd, m, cen, rad)) {
+                if (hit_box(ro, rd, m, par, cen, rad)) {
                     beam_radiance<T, LAYOUT>(a, 0, ro, rd, L);
                     tested++;
                 }
@@ -336,8 +340,8 @@ __global__ void __launch_bounds__(GATHER_THREADS) gather_kernel(cr::GatherArgs a
                     }
                     visited++;
                     const cr::TravNode nd = a.nodes[cur];
-                    const bool hl = hit_box(ro, rd, m, nd.cen[0], nd.rad[0]);
-                    const bool hr = hit_box(ro, rd, m, nd.cen[1], nd.rad[1]);
+                    const bool hl = hit_box(ro, rd, m, par, nd.cen[0], nd.rad[0]);
+                    const bool hr = hit_box(ro, rd, m, par, nd.cen[1], nd.rad[1]);
                     const bool ll = (nd.leafMask & 1u) != 0, lr = (nd.leafMask & 2u) != 0;
                     // The node's hit leaves, left before right, through one copy
                     // of the beam test: a wave whose lanes hit only left leaves
