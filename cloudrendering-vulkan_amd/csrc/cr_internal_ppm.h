@@ -45,6 +45,7 @@ struct PpmEstimateArgs {
     const uint32_t* arrivals;   // per cell
     const uint32_t* start;      // per cell: first of its run in `photons`
     const cr_photon* photons;   // deposits sorted by cell, emission order within a cell
+    const float4* dirs;         // their unit directions from (theta, phi), formed when they are sorted
     const int32_t* frames;      // (seed, frameCount, pmRadius bits) per frame
     uint32_t nFrames;
     float4* image;
@@ -54,7 +55,7 @@ struct PpmEstimateArgs {
 
 hipError_t launch_ppm_trace(const PpmTraceArgs& a, bool write, int dtype, int layout, hipStream_t s);
 hipError_t launch_ppm_runs(const uint32_t* keys, const uint32_t* vals, uint32_t n, uint32_t* start, uint32_t* arrivals,
-                           const cr_photon* photons, cr_photon* sorted, hipStream_t s);
+                           const cr_photon* photons, cr_photon* sorted, float4* dirs, hipStream_t s);
 hipError_t launch_ppm_estimate(const PpmEstimateArgs& a, int dtype, int layout, hipStream_t s);
 
 } // namespace cr

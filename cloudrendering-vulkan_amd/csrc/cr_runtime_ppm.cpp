@@ -24,6 +24,8 @@ struct Ppm {
     size_t pathCountsCap = 0, pathOffsetsCap = 0;
     cr_photon *deposits = nullptr, *sorted = nullptr;
     size_t depositsCap = 0, sortedCap = 0;
+    float4* dirs = nullptr;  // the sorted photons' unit directions
+    size_t dirsCap = 0;
     uint32_t *keys = nullptr, *vals = nullptr, *keysTmp = nullptr, *valsTmp = nullptr, *sortCounts = nullptr;
     size_t keysCap = 0, valsCap = 0, keysTmpCap = 0, valsTmpCap = 0, sortCountsCap = 0;
     uint32_t *arrivals = nullptr, *start = nullptr;
@@ -40,8 +42,8 @@ void ppm_free(cr_context* ctx)
 {
     Ppm* p = ctx->ppm;
     if (!p) return;
-    void* bufs[] = {p->pathCounts, p->pathOffsets, p->deposits, p->sorted, p->keys, p->vals, p->keysTmp, p->valsTmp,
-                    p->sortCounts, p->arrivals, p->start, p->frames};
+    void* bufs[] = {p->pathCounts, p->pathOffsets, p->deposits, p->sorted, p->dirs, p->keys, p->vals, p->keysTmp,
+                    p->valsTmp, p->sortCounts, p->arrivals, p->start, p->frames};
     for (void* b : bufs)
         if (b) hipFree(b);
     delete p;
@@ -199,6 +201,7 @@ cr_status cr_ppm_trace(cr_context* ctx, int32_t seed, const cr_ppm_params* prm, 
     if (nd > 0) {
         CR_TRY(ensure_buffer(ctx, (void**)&p->deposits, &p->depositsCap, (size_t)nd * sizeof(cr_photon), "deposits"));
         CR_TRY(ensure_buffer(ctx, (void**)&p->sorted, &p->sortedCap, (size_t)nd * sizeof(cr_photon), "sorted photons"));
+        CR_TRY(ensure_buffer(ctx, (void**)&p->dirs, &p->dirsCap, (size_t)nd * sizeof(float4), "photon directions"));
         CR_TRY(ensure_buffer(ctx, (void**)&p->keys, &p->keysCap, (size_t)nd * 4, "deposit cells"));
         CR_TRY(ensure_buffer(ctx, (void**)&p->vals, &p->valsCap, (size_t)nd * 4, "deposit order"));
         CR_TRY(ensure_buffer(ctx, (void**)&p->keysTmp, &p->keysTmpCap, (size_t)nd * 4, "sort keys"));
@@ -213,7 +216,8 @@ cr_status cr_ppm_trace(cr_context* ctx, int32_t seed, const cr_ppm_params* prm, 
         a.order = p->vals;
         CR_HIP(ctx, launch_ppm_trace(a, true, g.dtype, g.layout, s), "photon trace (write)");
         CR_HIP(ctx, launch_radix_sort(p->keys, p->vals, p->keysTmp, p->valsTmp, p->sortCounts, nd, s), "sort by cell");
-        CR_HIP(ctx, launch_ppm_runs(p->keys, p->vals, nd, p->start, p->arrivals, p->deposits, p->sorted, s), "cell runs");
+        CR_HIP(ctx, launch_ppm_runs(p->keys, p->vals, nd, p->start, p->arrivals, p->deposits, p->sorted, p->dirs, s),
+               "cell runs");
     }
     CR_HIP(ctx, hipStreamSynchronize(s), "sync");
     p->nDeposits = nd;
@@ -262,6 +266,7 @@ cr_status cr_ppm_estimate(cr_context* ctx, const cr_push_constants* frames, uint
     a.arrivals = p->arrivals;
     a.start = p->start;
     a.photons = p->sorted;
+    a.dirs = p->dirs;
     a.frames = p->frames;
     a.nFrames = n_frames;
     a.image = (float4*)ctx->image;

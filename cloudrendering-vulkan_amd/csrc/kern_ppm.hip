@@ -153,15 +153,23 @@ __global__ void __launch_bounds__(256) ppm_run_start_kernel(const uint32_t* keys
     if (i == 0 || keys[i - 1] != keys[i]) start[keys[i]] = i;
 }
 
+// The sorted copy of each deposit and its unit direction (sin theta cos phi,
+// sin theta sin phi, cos theta) — dirFromPolar, PPM_PE.comp:201-207 — formed once here
+// instead of by every estimate query that finds the photon in range.
 __global__ void __launch_bounds__(256) ppm_run_end_kernel(const uint32_t* keys, const uint32_t* vals, uint32_t n,
                                                           const uint32_t* start, uint32_t* arrivals,
-                                                          const cr_photon* photons, cr_photon* sorted)
+                                                          const cr_photon* photons, cr_photon* sorted, float4* dirs)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     const uint32_t c = keys[i];
     if (i + 1 == n || keys[i + 1] != c) arrivals[c] = i + 1 - start[c];
-    sorted[i] = photons[vals[i]];
+    const cr_photon ph = photons[vals[i]];
+    sorted[i] = ph;
+    float st, ct, sp, cph;
+    crd_sincosf(ph.theta, &st, &ct);
+    crd_sincosf(ph.phi, &sp, &cph);
+    dirs[i] = make_float4(st * cph, st * sp, ct, 0.0f);
 }
 
 // samplePhotonMap — PPM_PE.comp:304-375: photons within r of p, weighted by
@@ -190,10 +198,8 @@ __device__ __forceinline__ f4 photon_query(const cr::PpmEstimateArgs& a, f3 p, f
                     const cr_photon& ph = a.photons[s + i];
                     const f3 d = sub(ld3(ph.position), p);
                     if (dot(d, d) <= r2) {
-                        float st, ct, sp, cph;
-                        crd_sincosf(ph.theta, &st, &ct);
-                        crd_sincosf(ph.phi, &sp, &cph);
-                        const f3 pd = mk(st * cph, st * sp, ct);
+                        const float4 dv = a.dirs[s + i];
+                        const f3 pd = mk(dv.x, dv.y, dv.z);
                         const float f = phase_pdf(a.params, dir, pd);
                         acc.x = acc.x + f * ph.power[0];
                         acc.y = acc.y + f * ph.power[1];
@@ -232,6 +238,8 @@ __device__ __forceinline__ f4 estimate(const cr::PpmEstimateArgs& a, int32_t x, 
     const float albedo = 1.0f - pm.absorption;
     const f3 L = ld3(a.svp.lightDirection);
     const float dsbs = fdiv(cp.densityScaling, cp.baseScaling);
+    // the direct light's phase term: the same for every step of the ray
+    const float pdf = phase_pdf(a.params, neg(dir), L);
     GridView sv;
     sv.data = a.sv; sv.nx = a.svx; sv.ny = a.svy; sv.nz = a.svz; sv.nbx = 0; sv.nby = 0;
     float cur = 0.0f;
@@ -241,7 +249,6 @@ __device__ __forceinline__ f4 estimate(const cr::PpmEstimateArgs& a, int32_t x, 
         const float ext = fdiv(density<T, LAYOUT>(a.g, cp, pos) * cp.densityScaling, cp.baseScaling);
         const float sc = albedo * ext;
         // direct sunlight through the shadow volume
-        const float pdf = phase_pdf(a.params, neg(dir), L);
         const float tsun = trilinear<float, LAYOUT_DENSE>(sv, sv_norm(a.svp, pos));
         const float dl = ((1.0f * a.params.lightIntensity) * pdf) * tsun;
         f4 direct = {dl, dl, dl, dl};
@@ -323,12 +330,12 @@ hipError_t launch_ppm_trace(const PpmTraceArgs& a, bool write, int dtype, int la
 }
 
 hipError_t launch_ppm_runs(const uint32_t* keys, const uint32_t* vals, uint32_t n, uint32_t* start, uint32_t* arrivals,
-                           const cr_photon* photons, cr_photon* sorted, hipStream_t s)
+                           const cr_photon* photons, cr_photon* sorted, float4* dirs, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255u) / 256u);
     hipLaunchKernelGGL(ppm_run_start_kernel, grid, dim3(256), 0, s, keys, n, start);
-    hipLaunchKernelGGL(ppm_run_end_kernel, grid, dim3(256), 0, s, keys, vals, n, start, arrivals, photons, sorted);
+    hipLaunchKernelGGL(ppm_run_end_kernel, grid, dim3(256), 0, s, keys, vals, n, start, arrivals, photons, sorted, dirs);
     return hipGetLastError();
 }
 
