@@ -470,13 +470,15 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                             float4 v = J.image[(size_t)y * J.W + x];
                             L.acc.x = v.x; L.acc.y = v.y; L.acc.z = v.z; L.acc.w = v.w;
                         }
-                        f3 cp, cd;
-                        camera_ray(J.cam, (float)(x - J.cam.halfWidth), (float)(y - J.cam.halfHeight), cp, cd);
-                        float tmax = 0.0f, tmin = 0.0f;
-                        L.camMiss = !intersect_box(B0(J), B1(J), cp, cd, tmin, tmax) || tmax < 0.0f ||
-                                    J.cloud.densityScaling <= 0.0f;
-                        L.camPos = L.camMiss ? cp : add(cp, mul(cd, tmax < tmin ? tmax : tmin));
-                        L.camDir = cd;
+                        if (!BATCH) {
+                            f3 cp, cd;
+                            camera_ray(J.cam, (float)(x - J.cam.halfWidth), (float)(y - J.cam.halfHeight), cp, cd);
+                            float tmax = 0.0f, tmin = 0.0f;
+                            L.camMiss = !intersect_box(B0(J), B1(J), cp, cd, tmin, tmax) || tmax < 0.0f ||
+                                        J.cloud.densityScaling <= 0.0f;
+                            L.camPos = L.camMiss ? cp : add(cp, mul(cd, tmax < tmin ? tmax : tmin));
+                            L.camDir = cd;
+                        }
                         if (J.nFrames > 0u) {
                             L.seedNext = J.frames[0];
                             L.countNext = J.frames[1];
@@ -522,6 +524,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
             // blend: sky = the path left the box (camera rays that miss it take
             // the background as their radiance), fin = the sample is complete.
             bool sky = false, missed = false, fin = false;
+            bool cam = false;  // batched: a camera ray, its box test after the shared reciprocal below
             if (needSample) {
                 needSample = false;
                 if (L.f >= (KMODE == K_CONTINUE ? a.spp : a.nFrames)) {
@@ -563,8 +566,10 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                     ev = EV_COLLIDE;
                     continue;
                 }
-                // start frame f: PathTracer.comp:368-392 (the camera ray and its box
-                // test were made when the lane took the pixel)
+                // start frame f: PathTracer.comp:368-392 (a single render made the
+                // camera ray and its box test when the lane took the pixel; the
+                // batched kernel, whose lanes take pixels of small jobs with few
+                // frames, measures faster forming them per frame)
                 const int32_t seed = L.seedNext;
                 L.count = (uint32_t)L.countNext;
                 if (L.f + 1u < a.nFrames) {
@@ -572,15 +577,22 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                     L.countNext = a.frames[2 * (L.f + 1u) + 1];
                 }
                 xs_init(L.rng, (uint32_t)seed * L.id);
-                L.pos = L.camPos;
-                L.dir = L.camDir;
+                if (BATCH) {
+                    camera_ray(a.cam, (float)(L.px - a.cam.halfWidth), (float)(L.py - a.cam.halfHeight), L.pos, L.dir);
+                } else {
+                    L.pos = L.camPos;
+                    L.dir = L.camDir;
+                }
                 c.paths++;
                 c.vertices++;
                 L.res.x = 0.0f; L.res.y = 0.0f; L.res.z = 0.0f; L.res.w = 0.0f;
                 L.thr = 1.0f;
                 L.bounces = 0;
                 L.mode = MODE_PRIMARY;
-                if (L.camMiss) {
+                if (BATCH) {
+                    cam = true;
+                    seg = 1;
+                } else if (L.camMiss) {
                     sky = true;
                     missed = true;
                 } else {
@@ -638,6 +650,23 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 L.dir = scatter(a.params, L.dir, L.rng);
                 seg = 1;
             }
+            // the direction of the segment each lane starts; the batched kernel
+            // forms its reciprocal here for the camera rays' box test as well
+            const f3 sd = seg == 2 ? sdir : L.dir;
+            f3 invd;
+            if (BATCH) {
+                invd = recip(sd);
+                if (cam) {
+                    float tmax = 0.0f, tmin = 0.0f;
+                    if (!intersect_box_inv(B0(a), B1(a), L.pos, invd, tmin, tmax) || tmax < 0.0f ||
+                        a.cloud.densityScaling <= 0.0f) {
+                        sky = true;
+                        missed = true;
+                    } else {
+                        L.pos = add(L.pos, mul(L.dir, tmax < tmin ? tmax : tmin));
+                    }
+                }
+            }
             if (sky) {
                 const f4 bg = background(a.ext, L.dir);
                 if (missed) {
@@ -655,8 +684,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 continue;
             }
             // seg != 0 here: every branch that falls through starts a segment
-            const f3 sd = seg == 2 ? sdir : L.dir;
-            ev = begin_segment<TRACK>(a, L, sd, recip(sd)) ? EV_NONE : EV_EXIT;
+            ev = begin_segment<TRACK>(a, L, sd, BATCH ? invd : recip(sd)) ? EV_NONE : EV_EXIT;
         }
     }
 
