@@ -650,12 +650,11 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 L.dir = scatter(a.params, L.dir, L.rng);
                 seg = 1;
             }
-            // the direction of the segment each lane starts; the batched kernel
-            // forms its reciprocal here for the camera rays' box test as well
+            // the direction of the segment each lane starts and its reciprocal
+            // (the batched kernel's camera rays take their box test with it)
             const f3 sd = seg == 2 ? sdir : L.dir;
-            f3 invd;
+            const f3 invd = recip(sd);
             if (BATCH) {
-                invd = recip(sd);
                 if (cam) {
                     float tmax = 0.0f, tmin = 0.0f;
                     if (!intersect_box_inv(B0(a), B1(a), L.pos, invd, tmin, tmax) || tmax < 0.0f ||
@@ -684,7 +683,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                 continue;
             }
             // seg != 0 here: every branch that falls through starts a segment
-            ev = begin_segment<TRACK>(a, L, sd, BATCH ? invd : recip(sd)) ? EV_NONE : EV_EXIT;
+            ev = begin_segment<TRACK>(a, L, sd, invd) ? EV_NONE : EV_EXIT;
         }
     }
 
