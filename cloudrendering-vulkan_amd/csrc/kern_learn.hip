@@ -40,8 +40,35 @@ __global__ void __launch_bounds__(256) mip_kernel(const float* src, int32_t sx, 
     dst[(size_t)x + (size_t)dx * ((size_t)y + (size_t)dy * z)] = fdiv(acc, cnt);
 }
 
+// The coarsest mip levels, as many as fit in this many floats (32 KB), are
+// copied into LDS by each workgroup: for a 128^3 grid, levels 3-7 (16^3 down
+// to 1^3, 18.3 KB), whose taps are 135 of a record's 216.
+constexpr uint32_t FEATURE_LDS_FLOATS = 8192;
+
+__device__ __forceinline__ uint32_t mip_floats(const cr::FeatureArgs& a, int k)
+{
+    return (uint32_t)a.dims[k][0] * (uint32_t)a.dims[k][1] * (uint32_t)a.dims[k][2];
+}
+
 __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
 {
+    __shared__ float mipLds[FEATURE_LDS_FLOATS];
+    // levels staged..LEARN_LEVELS-1 live in LDS, in that order
+    int staged = cr::LEARN_LEVELS;
+    uint32_t used = 0;
+    while (staged > 0 && used + mip_floats(a, staged - 1) <= FEATURE_LDS_FLOATS) {
+        staged--;
+        used += mip_floats(a, staged);
+    }
+    {
+        uint32_t off = 0;
+        for (int k = staged; k < cr::LEARN_LEVELS; k++) {
+            const uint32_t nk = mip_floats(a, k);
+            for (uint32_t v = threadIdx.x; v < nk; v += 256u) mipLds[off + v] = a.mips[k][v];
+            off += nk;
+        }
+    }
+    __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (a.ndev ? (uint64_t)*a.ndev : a.n)) return;
     const float* pt = a.points + i * 8;
@@ -61,6 +88,7 @@ __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
     // the last with the two cosines and padding), so few values stay live
     uint32_t* out = reinterpret_cast<uint32_t*>(a.X + i * 224);
     float h = a.h0;
+    uint32_t ldsOff = 0;  // the next staged level's place in mipLds
 #pragma unroll 1
     for (int grp = 0; grp < 4; grp++) {
         uint32_t packed[28];
@@ -69,7 +97,12 @@ __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
         for (int kk = 0; kk < 2; kk++) {
             const int k = 2 * grp + kk;
             GridView g;
-            g.data = a.mips[k];
+            if (k >= staged) {
+                g.data = mipLds + ldsOff;
+                ldsOff += mip_floats(a, k);
+            } else {
+                g.data = a.mips[k];
+            }
             g.nx = a.dims[k][0]; g.ny = a.dims[k][1]; g.nz = a.dims[k][2];
             g.nbx = 0; g.nby = 0;
 #pragma unroll
