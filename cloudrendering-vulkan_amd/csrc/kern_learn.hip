@@ -97,31 +97,36 @@ __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
         for (int kk = 0; kk < 2; kk++) {
             const int k = 2 * grp + kk;
             GridView g;
+            g.nx = a.dims[k][0]; g.ny = a.dims[k][1]; g.nz = a.dims[k][2];
+            g.nbx = 0; g.nby = 0;
+            // the 27 taps of level k; a staged level's copy reads LDS directly
+            auto taps = [&](const GridView& gv) {
+#pragma unroll
+                for (int l = -1; l <= 1; l++)
+#pragma unroll
+                    for (int j = -1; j <= 1; j++)
+#pragma unroll
+                        for (int ii = -1; ii <= 1; ii++) {
+                            const float fi = (float)ii * h, fj = (float)j * h, fl = (float)l * h;
+                            const f3 p = mk(x.x + ((fi * ex.x + fj * ey.x) + fl * ez.x),
+                                            x.y + ((fi * ex.y + fj * ey.y) + fl * ez.y),
+                                            x.z + ((fi * ex.z + fj * ey.z) + fl * ez.z));
+                            float d = 0.0f;
+                            if (in_box(b0, b1, p)) d = trilinear<float, LAYOUT_DENSE>(gv, cloud_norm(a.cloud, p));
+                            const uint16_t v = f2bf((d * sscale) * h);
+                            if (idx & 1) packed[idx >> 1] |= (uint32_t)v << 16;
+                            else packed[idx >> 1] = v;
+                            idx++;
+                        }
+            };
             if (k >= staged) {
                 g.data = mipLds + ldsOff;
                 ldsOff += mip_floats(a, k);
+                taps(g);
             } else {
                 g.data = a.mips[k];
+                taps(g);
             }
-            g.nx = a.dims[k][0]; g.ny = a.dims[k][1]; g.nz = a.dims[k][2];
-            g.nbx = 0; g.nby = 0;
-#pragma unroll
-            for (int l = -1; l <= 1; l++)
-#pragma unroll
-                for (int j = -1; j <= 1; j++)
-#pragma unroll
-                    for (int ii = -1; ii <= 1; ii++) {
-                        const float fi = (float)ii * h, fj = (float)j * h, fl = (float)l * h;
-                        const f3 p = mk(x.x + ((fi * ex.x + fj * ey.x) + fl * ez.x),
-                                        x.y + ((fi * ex.y + fj * ey.y) + fl * ez.y),
-                                        x.z + ((fi * ex.z + fj * ey.z) + fl * ez.z));
-                        float d = 0.0f;
-                        if (in_box(b0, b1, p)) d = trilinear<float, LAYOUT_DENSE>(g, cloud_norm(a.cloud, p));
-                        const uint16_t v = f2bf((d * sscale) * h);
-                        if (idx & 1) packed[idx >> 1] |= (uint32_t)v << 16;
-                        else packed[idx >> 1] = v;
-                        idx++;
-                    }
             h = h * 2.0f;
         }
         if (grp == 3) {
