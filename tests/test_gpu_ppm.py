@@ -44,16 +44,27 @@ def oracle_map(O, ctx, sc, svp, pm, p, seed):
     return s, n, arrivals, start, sorted_
 
 
-@pytest.mark.parametrize("variant", ["f32_dense", "f16_brick", "coarse"])
+@pytest.mark.parametrize("variant", ["f32_dense", "f16_brick", "coarse", "long_paths"])
 def test_ppm_trace_bitexact(ctx, cfg, S, O, pkg, variant):
     """coarse: a 12^3 photon grid, where many cells receive more than the 32
-    photons they keep."""
+    photons they keep.  long_paths: a 30 times denser cloud and no absorption,
+    so paths scatter until they leave the box, with more than 64 deposits per
+    path on average."""
     storage = (S.CR_DTYPE_F16, S.CR_GRID_BRICK4) if variant == "f16_brick" else (None, None)
     sc, cloud, svp, pm, p = setup(ctx, cfg, S, pkg, *storage, cells=12 if variant == "coarse" else 100)
+    if variant == "long_paths":
+        cloud.densityScaling *= 30.0
+        ctx.set_cloud(cloud)
+        ctx.sv_build()
+        pm.absorption = 0.0
+        ctx.set_photon_map(pm)
+        p.numPaths = 400
     n = ctx.ppm_trace(38, p)
     arrivals, photons, nd = ctx.ppm_download()
     s, rn, rarr, rstart, rsorted = oracle_map(O, ctx, sc, svp, pm, p, 38)
     assert nd == n == rn and n > 10000
+    if variant == "long_paths":
+        assert n > 64 * p.numPaths
     np.testing.assert_array_equal(arrivals, rarr)
     assert bytes(photons)[:48 * n] == bytes(rsorted)[:48 * n], "photon grid differs"
     if variant == "coarse":
