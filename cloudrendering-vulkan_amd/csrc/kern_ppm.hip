@@ -258,11 +258,25 @@ __device__ __forceinline__ f4 estimate(const cr::PpmEstimateArgs& a, int32_t x, 
         float accD = 0.0f, accL = 0.0f;
         uint32_t samples = 0;
         f4 amb = background(a.ext, ad);
-        for (uint32_t j = 0; j < MARCH_CEILING && in_box(cp.bounds[0], cp.bounds[1], q); j++) {
-            q = add(q, mul(ad, a.ambientStep));
-            accD = accD + density<T, LAYOUT>(a.g, cp, q);
+        // Two samples per iteration, their lookups issued together: the second
+        // is taken only if the point before it is still inside the box, as
+        // the one-sample loop would (a sample outside the box reads clamped
+        // texels, so the speculative lookup stays in bounds).  Same points,
+        // same sums in the same order.
+        const f3 dq = mul(ad, a.ambientStep);
+        for (uint32_t j = 0; j < MARCH_CEILING && in_box(cp.bounds[0], cp.bounds[1], q); j += 2) {
+            const f3 q1 = add(q, dq);
+            const f3 q2 = add(q1, dq);
+            const float v1 = density<T, LAYOUT>(a.g, cp, q1);
+            const float v2 = density<T, LAYOUT>(a.g, cp, q2);
+            accD = accD + v1;
             accL = accL + a.ambientStep;
             samples++;
+            if (!(j + 1 < MARCH_CEILING && in_box(cp.bounds[0], cp.bounds[1], q1))) break;
+            accD = accD + v2;
+            accL = accL + a.ambientStep;
+            samples++;
+            q = q2;
         }
         accD = accD * dsbs;
         const float ta = crd_expf(fdiv(-accL * accD, (float)samples));
