@@ -6,6 +6,7 @@
   features of the recorded collisions, in record order;
 * with shadow-volume NEE the first collision's next-event term is left out of
   the targets and added back exactly by the learned render (bit for bit);
+* a render of n frames in one call equals the same frames over several calls;
 * stencil features against the NumPy oracle (bf16 tolerance);
 * the gain: a network trained on path-traced samples, rendered at 4 spp, has
   a lower relMSE against a converged path-traced reference than the plain
@@ -137,6 +138,27 @@ def test_shadow_volume_first_nee_is_exact(ctx, cfg, S, pkg):
     ctx.set_render_ext(sc.ext)
     assert ctx.lib.cr_learn_collect(ctx.h, (S.PushConstants * 2)(*fr[:2]), 2, 2, 0, None) != 0
     ctx.learn_dataset_clear()
+
+
+def test_learned_render_is_the_same_in_any_number_of_calls(ctx, cfg, S):
+    """n frames of a learned render in one call give the image that the same
+    frames give over several calls: the composition blends frame by frame in
+    frame order, and a record's features and network output do not depend on
+    the other records of its call (renderers pick the frames per call for
+    speed alone)."""
+    sc = scene(cfg, S, seed=5)
+    sc.setup(ctx)
+    ctx.mlp_init(3)
+    fr = cfg.frames(12)
+    ctx.clear_image()
+    ctx.learn_render(fr)
+    one = ctx.download_image(sc.height, sc.width)
+    ctx.clear_image()
+    for a, b in ((0, 5), (5, 6), (6, 12)):
+        ctx.learn_render(fr[a:b])
+    parts = ctx.download_image(sc.height, sc.width)
+    assert one[..., :3].max() > 0.0
+    assert_bitexact(parts, one, "learned render of 12 frames in three calls vs one")
 
 
 def test_features_match_oracle(ctx, cfg, S):
