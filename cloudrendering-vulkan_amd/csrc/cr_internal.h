@@ -76,9 +76,15 @@ struct BatchArgs {
     unsigned long long* counters;
 };
 
-// kern_pt.hip
+// kern_pt.hip: a render of a.nFrames frames; launches of at most
+// PT_FEW_FRAMES frames use the instantiation with gated pixel claims
+// (kern_pt_few.hip), which is faster when each pixel carries little work.
+constexpr uint32_t PT_FEW_FRAMES = 16;
 hipError_t launch_pt(const PTArgs& a, int dtype, int layout, int grid_blocks, hipStream_t s);
-int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee);
+int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, uint32_t n_frames);
+// kern_pt_few.hip
+hipError_t launch_pt_few(const PTArgs& a, int dtype, int layout, int grid_blocks, hipStream_t s);
+int pt_few_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee);
 hipError_t launch_pt_batch(const BatchArgs& b, int dtype, int layout, uint32_t tracking, uint32_t nee, int grid_blocks,
                            int block_threads, hipStream_t s);
 int pt_batch_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, int block_threads, uint32_t n_jobs);

@@ -738,7 +738,7 @@ cr_status cr_pt_render(cr_context* ctx, const cr_push_constants* frames, uint32_
     PTArgs a;
     fill_pt_args(ctx, a, n_frames);
     const DeviceGrid& g = ctx->grid;
-    int per_cu = pt_blocks_per_cu(g.dtype, g.layout, ctx->ext.trackingMode, ctx->ext.neeMode);
+    int per_cu = pt_blocks_per_cu(g.dtype, g.layout, ctx->ext.trackingMode, ctx->ext.neeMode, n_frames);
     int blocks = ctx->numCUs * per_cu;
     uint32_t needed = (a.totalIdx + 255u) / 256u;
     if ((uint32_t)blocks > needed) blocks = (int)needed;

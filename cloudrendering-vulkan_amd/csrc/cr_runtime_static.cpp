@@ -107,7 +107,7 @@ cr_status cr_pt_render_static(cr_context* ctx, const cr_push_constants* frames, 
     a.tileOrder = nullptr;
     a.tileCost = nullptr;
     const DeviceGrid& g = ctx->grid;
-    int blocks = ctx->numCUs * pt_blocks_per_cu(g.dtype, g.layout, ctx->ext.trackingMode, ctx->ext.neeMode);
+    int blocks = ctx->numCUs * pt_blocks_per_cu(g.dtype, g.layout, ctx->ext.trackingMode, ctx->ext.neeMode, n_frames);
     const uint32_t needed = (a.totalIdx + 255u) / 256u;
     if ((uint32_t)blocks > needed) blocks = (int)needed;
     CR_HIP(ctx, launch_pt(a, g.dtype, g.layout, blocks, ctx->stream), "path tracer kernel");

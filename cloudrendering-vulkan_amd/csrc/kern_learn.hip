@@ -53,6 +53,12 @@ __device__ __forceinline__ uint32_t mip_floats(const cr::FeatureArgs& a, int k)
 __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
 {
     __shared__ float mipLds[FEATURE_LDS_FLOATS];
+    // The grid is sized for every record; the count of collisions among them
+    // is known only on the device.  A workgroup wholly past it leaves before
+    // staging (the test is the same for all its threads, so no thread is left
+    // at the barrier).
+    const uint64_t n = a.ndev ? (uint64_t)*a.ndev : a.n;
+    if ((uint64_t)blockIdx.x * 256 >= n) return;
     // levels staged..LEARN_LEVELS-1 live in LDS, in that order
     int staged = cr::LEARN_LEVELS;
     uint32_t used = 0;
@@ -70,7 +76,7 @@ __global__ void __launch_bounds__(256) features_kernel(cr::FeatureArgs a)
     }
     __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (a.ndev ? (uint64_t)*a.ndev : a.n)) return;
+    if (i >= n) return;
     const float* pt = a.points + i * 8;
     const f3 x = mk(pt[0], pt[1], pt[2]);
     const f3 w = mk(pt[3], pt[4], pt[5]);

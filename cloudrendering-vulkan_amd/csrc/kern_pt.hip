@@ -29,12 +29,14 @@ hipError_t launch_tile_order(const uint32_t* cost, uint32_t n, uint32_t* keys, u
 
 hipError_t launch_pt(const PTArgs& a, int dtype, int layout, int blocks, hipStream_t s)
 {
+    if (a.nFrames <= PT_FEW_FRAMES) return launch_pt_few(a, dtype, layout, blocks, s);
     BatchArgs none = {};
     return launch(pick<false, K_RENDER>(dtype, layout, a.ext.trackingMode, a.ext.neeMode), a, none, blocks, 256, 0, s);
 }
 
-int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee)
+int pt_blocks_per_cu(int dtype, int layout, uint32_t tracking, uint32_t nee, uint32_t n_frames)
 {
+    if (n_frames <= PT_FEW_FRAMES) return pt_few_blocks_per_cu(dtype, layout, tracking, nee);
     return occupancy(pick<false, K_RENDER>(dtype, layout, tracking, nee), 256, 0);
 }
 

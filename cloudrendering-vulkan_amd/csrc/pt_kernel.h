@@ -339,6 +339,9 @@ __device__ __forceinline__ void blend(const cr::PTArgs& a, Lane& L)
 
 // What a launch computes per work item.
 //   K_RENDER   : pixels, frames blended into the image (the reference's pass).
+//   K_RENDER_FEW: the same for launches of a few frames per pixel (the
+//                reference's interactive loop renders one), with claims gated
+//                as in K_RECORD.
 //   K_RECORD   : pixels; each frame's camera path stops at its first real
 //                collision, which is recorded as (position, direction, RNG
 //                state, flag 1), or, for a path that escapes before any
@@ -347,7 +350,18 @@ __device__ __forceinline__ void blend(const cr::PTArgs& a, Lane& L)
 //                collision (NEE, scattering, further bounces); the first
 //                resumes the recorded RNG state, so RECORD + CONTINUE(1)
 //                reproduces K_RENDER's sample exactly.  Output: the mean.
-enum { K_RENDER = 0, K_RECORD = 1, K_CONTINUE = 2 };
+enum { K_RENDER = 0, K_RECORD = 1, K_CONTINUE = 2, K_RENDER_FEW = 3 };
+
+constexpr bool renders(int kmode) { return kmode == K_RENDER || kmode == K_RENDER_FEW; }
+
+// Passes whose pixels carry little work (a few frames, or camera paths cut at
+// their first collision) claim pixels only once CLAIM_MIN_LANES lanes of the
+// wave (or all of its live ones) are idle: lanes finish such pixels at
+// scattered times, and the claim (queue atomic, pixel set-up, camera ray and
+// box test) would otherwise run for one or two lanes nearly every pass.
+// Launches of many frames per pixel claim as soon as one lane is idle.
+constexpr bool gated_claims(int kmode) { return kmode == K_RECORD || kmode == K_RENDER_FEW; }
+constexpr uint32_t CLAIM_MIN_LANES = 16;
 
 __device__ __forceinline__ float* record_of(const cr::PTArgs& a, const Lane& L)
 {
@@ -367,7 +381,7 @@ __device__ __forceinline__ uint32_t continuation_seed(uint32_t s0, uint32_t s)
 template <int KMODE>
 __device__ __forceinline__ void finish_sample(const cr::PTArgs& a, Lane& L)
 {
-    if (KMODE == K_RENDER) {
+    if (renders(KMODE)) {
         blend(a, L);
     } else if (KMODE == K_CONTINUE) {
         L.acc.x = L.acc.x + L.res.x;
@@ -431,7 +445,8 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
     for (;;) {
         // ---- claim pixels for idle lanes (wave-wide, one atomic per wave)
         const unsigned long long idle = __ballot(state == ST_IDLE);
-        if (idle) {
+        if (idle && (!gated_claims(KMODE) ||
+                     (uint32_t)__popcll(idle) >= min(CLAIM_MIN_LANES, (uint32_t)__popcll(__ballot(state != ST_DEAD))))) {
             uint32_t base = 0;
             const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
             if (lane == leader) base = atomicAdd(queue, (uint32_t)__popcll(idle));
@@ -466,7 +481,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
                         L.px = x;
                         L.py = y;
                         L.id = (uint32_t)x + (uint32_t)y * J.padded;
-                        if (KMODE == K_RENDER) {
+                        if (renders(KMODE)) {
                             float4 v = J.image[(size_t)y * J.W + x];
                             L.acc.x = v.x; L.acc.y = v.y; L.acc.z = v.z; L.acc.w = v.w;
                         }
@@ -528,7 +543,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
             if (needSample) {
                 needSample = false;
                 if (L.f >= (KMODE == K_CONTINUE ? a.spp : a.nFrames)) {
-                    if (KMODE == K_RENDER) {
+                    if (renders(KMODE)) {
                         float4 o;
                         o.x = L.acc.x; o.y = L.acc.y; o.z = L.acc.z; o.w = L.acc.w;
                         a.image[(size_t)L.py * a.W + L.px] = o;

@@ -205,16 +205,19 @@ def c2_small(cfg, w=48, h=28, size=(64, 32, 64)):
     return sc
 
 
-def test_pt_c2_shape_bitexact(ctx, cfg, S, O):
+@pytest.mark.parametrize("n_frames", [4, 20], ids=["few_frame_launch", "many_frame_launch"])
+def test_pt_c2_shape_bitexact(ctx, cfg, S, O, n_frames):
     """The benchmarked kernel instantiation (FP16 + 4^3 bricks + brick DDA +
     ratio tracking) with the benchmark's albedo, bounce cap and generator,
-    against the oracle (PathTracer.comp:318-361, 366-425 plus the extensions)."""
+    against the oracle (PathTracer.comp:318-361, 366-425 plus the extensions).
+    Launches of at most 16 frames run the instantiation with gated pixel
+    claims, longer ones (the benchmark's 256) the other: both are checked."""
     sc = c2_small(cfg)
     assert sc.storage == S.CR_DTYPE_F16 and sc.grid_layout == S.CR_GRID_BRICK4
     assert sc.ext.trackingMode == S.CR_TRACK_BRICK_DDA and sc.ext.neeMode == S.CR_NEE_RATIO_TRACKING
     assert sc.ext.bounceCap == 256 and abs(sc.ext.albedo - 0.999) < 1e-7
     cloud, svp = sc.setup(ctx)
-    fr = cfg.frames(4)
+    fr = cfg.frames(n_frames)
     gpu, ref, st, cnt = render_both(ctx, O, sc, svp, fr)
     assert_bitexact(gpu, ref, "c2-shaped image")
     assert (st.paths, st.vertices, st.lookups, st.svLookups, st.brickVisits) == \
@@ -257,10 +260,12 @@ def test_pt_c2_full_size_region_bitexact(ctx, cfg, S, O):
 
 
 def test_pt_frames_split_equals_single_launch(ctx, cfg, S):
-    """n frames in one persistent launch == the same frames one launch each."""
+    """n frames in one persistent launch == the same frames one launch each
+    (20 frames: the one launch runs the many-frame instantiation, the single
+    frames the one with gated pixel claims)."""
     sc = small_c1(cfg, S, w=32, h=32)
     sc.setup(ctx)
-    fr = cfg.frames(6)
+    fr = cfg.frames(20)
     ctx.clear_image()
     ctx.pt_render(fr)
     a = ctx.download_image(32, 32)
