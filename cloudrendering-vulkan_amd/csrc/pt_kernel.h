@@ -354,13 +354,14 @@ enum { K_RENDER = 0, K_RECORD = 1, K_CONTINUE = 2, K_RENDER_FEW = 3 };
 
 constexpr bool renders(int kmode) { return kmode == K_RENDER || kmode == K_RENDER_FEW; }
 
-// Passes whose pixels carry little work (a few frames, or camera paths cut at
-// their first collision) claim pixels only once CLAIM_MIN_LANES lanes of the
-// wave (or all of its live ones) are idle: lanes finish such pixels at
-// scattered times, and the claim (queue atomic, pixel set-up, camera ray and
-// box test) would otherwise run for one or two lanes nearly every pass.
-// Launches of many frames per pixel claim as soon as one lane is idle.
-constexpr bool gated_claims(int kmode) { return kmode == K_RECORD || kmode == K_RENDER_FEW; }
+// Passes whose pixels carry little work (a few frames, camera paths cut at
+// their first collision, or the small jobs of a batched launch) claim pixels
+// only once CLAIM_MIN_LANES lanes of the wave (or all of its live ones) are
+// idle: lanes finish such pixels at scattered times, and the claim (queue
+// atomic, job lookup, pixel set-up, camera ray and box test) would otherwise
+// run for one or two lanes nearly every pass.  Single renders of many frames
+// per pixel claim as soon as one lane is idle.
+constexpr bool gated_claims(int kmode, bool batch) { return batch || kmode == K_RECORD || kmode == K_RENDER_FEW; }
 constexpr uint32_t CLAIM_MIN_LANES = 16;
 
 __device__ __forceinline__ float* record_of(const cr::PTArgs& a, const Lane& L)
@@ -445,7 +446,7 @@ __global__ void __launch_bounds__(256) pt_kernel(cr::PTArgs a0, cr::BatchArgs b)
     for (;;) {
         // ---- claim pixels for idle lanes (wave-wide, one atomic per wave)
         const unsigned long long idle = __ballot(state == ST_IDLE);
-        if (idle && (!gated_claims(KMODE) ||
+        if (idle && (!gated_claims(KMODE, BATCH) ||
                      (uint32_t)__popcll(idle) >= min(CLAIM_MIN_LANES, (uint32_t)__popcll(__ballot(state != ST_DEAD))))) {
             uint32_t base = 0;
             const uint32_t leader = (uint32_t)__ffsll((long long)idle) - 1u;
